@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the slab C2C FFTND path on one MI355X.
+
+Workload (the metric's configuration): FFTND C2C complex128 512x512x512 at
+one rank.  One step = forward execute + backward execute + normalize through
+the reference-style C API (each call host-synchronous, as in SHAFFT).
+GFLOP/s counts 5 N log2 N per direction (2 directions per step).
+
+Also reported, on the same JSON line:
+  * roofline: the dominant kernel (the forward LDS pass kernel), its average
+    launch duration from HIP events recorded on the plan's stream around the
+    forward executes of the timed region, and its algorithmic bytes per
+    launch (one read + one write of the 2^27 complex128 grid = 4.295 GB);
+  * cpu_baseline: the oracle's FP64 FFT (radix-2, OpenMP) timed on this
+    host on one forward transform of the same grid.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import ctypes as ct
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "C2C GFLOP/s (5N log2 N) & % HBM peak, 512^3 fp64; served p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+class Hip:
+    """Minimal HIP runtime binding for streams and events (timing only)."""
+
+    def __init__(self):
+        self.L = ct.CDLL("libamdhip64.so")
+        L = self.L
+        L.hipStreamCreate.argtypes = [ct.POINTER(ct.c_void_p)]
+        L.hipEventCreate.argtypes = [ct.POINTER(ct.c_void_p)]
+        L.hipEventRecord.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.hipEventSynchronize.argtypes = [ct.c_void_p]
+        L.hipEventElapsedTime.argtypes = [ct.POINTER(ct.c_float), ct.c_void_p, ct.c_void_p]
+        L.hipDeviceSynchronize.argtypes = []
+        L.hipStreamSynchronize.argtypes = [ct.c_void_p]
+
+    def stream(self):
+        s = ct.c_void_p()
+        assert self.L.hipStreamCreate(ct.byref(s)) == 0
+        return s.value
+
+    def event(self):
+        e = ct.c_void_p()
+        assert self.L.hipEventCreate(ct.byref(e)) == 0
+        return e.value
+
+    def record(self, e, s):
+        assert self.L.hipEventRecord(ct.c_void_p(e), ct.c_void_p(s)) == 0
+
+    def elapsed_ms(self, a, b):
+        self.L.hipEventSynchronize(ct.c_void_p(b))
+        f = ct.c_float()
+        assert self.L.hipEventElapsedTime(ct.byref(f), ct.c_void_p(a), ct.c_void_p(b)) == 0
+        return f.value
+
+    def sync(self):
+        assert self.L.hipDeviceSynchronize() == 0
+
+
+def cpu_baseline(x, dims, prec):
+    import oracle  # the CPU-baseline leg is the only bench use of the oracle
+
+    a = x.reshape(dims).astype(np.complex128)
+    t0 = time.perf_counter()
+    oracle.fftn_pow2(a, None, -1)
+    dt = time.perf_counter() - t0
+    n = a.size
+    return {
+        "value": round(5 * n * math.log2(n) / dt / 1e9, 3),
+        "unit": "GFLOP/s",
+        "cores": oracle.threads(),
+        "kind": "port",
+        "sample": f"one forward FFTND of the full {'x'.join(map(str, dims))} grid in FP64 "
+                  f"({dt:.2f} s), oracle radix-2 FFT with OpenMP over lines; FFTW is not installed",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--dims", type=str, default="512,512,512")
+    ap.add_argument("--prec", type=str, default="z2z", choices=["c2c", "z2z"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU per process; multi-GPU runs "
+                                   "are not part of it (see DESIGN.md, sharding)"}))
+        sys.exit(2)
+
+    from shafft_amd import api
+
+    dims = tuple(int(v) for v in args.dims.split(","))
+    prec = api.Z2Z if args.prec == "z2z" else api.C2C
+    esz = 16 if prec == api.Z2Z else 8
+    n = int(np.prod(dims))
+    hip = Hip()
+    stream = hip.stream()
+
+    rng = np.random.default_rng(43)
+    x = np.empty(n, np.complex128 if prec == api.Z2Z else np.complex64)
+    x.real = rng.uniform(-1, 1, n)
+    x.imag = rng.uniform(-1, 1, n)
+
+    plan = api.FFTND()
+    plan.init([1] * len(dims), dims, prec)
+    plan.plan()
+    plan.set_stream(stream)
+    a, b = api.DeviceBuffer(n, prec), api.DeviceBuffer(n, prec)
+    a.upload(x)
+    plan.set_buffers(a.ptr, b.ptr)
+
+    # seam handle with the same pass program, for per-pass byte counts
+    seam = api.NDBackendPlan(dims, list(range(len(dims))), prec)
+    npass = seam.num_passes()
+    pass_bytes = [seam.pass_bytes(i) for i in range(npass)]
+    del seam
+
+    for _ in range(args.warmup):
+        plan.execute(api.FORWARD)
+        plan.execute(api.BACKWARD)
+        plan.normalize()
+    hip.sync()
+
+    ev = [(hip.event(), hip.event()) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        hip.record(ev[i][0], stream)
+        plan.execute(api.FORWARD)
+        hip.record(ev[i][1], stream)
+        plan.execute(api.BACKWARD)
+        plan.normalize()
+    hip.sync()
+    dt = time.perf_counter() - t0
+    fwd_ms = [hip.elapsed_ms(e0, e1) for e0, e1 in ev]
+
+    flops_step = 2 * 5 * n * math.log2(n)
+    ms_step = dt * 1e3 / args.steps
+    gflops = flops_step * args.steps / dt / 1e9
+    # dominant kernel: the forward pass kernel, npass launches per forward execute
+    avg_launch_ms = float(np.mean(fwd_ms)) / npass
+    bytes_launch = float(np.mean(pass_bytes))
+    achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
+    step_bytes = 2 * sum(pass_bytes) + 2 * n * esz  # fwd + bwd + normalize
+    out = {
+        "metric": METRIC,
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == api.Z2Z else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"FFTND C2C {'complex128' if prec == api.Z2Z else 'complex64'} "
+                        f"{'x'.join(map(str, dims))}, one rank; step = forward + backward + normalize",
+            "dims": list(dims),
+            "passes_per_direction": npass,
+            "forward_ms": round(float(np.mean(fwd_ms)), 4),
+            "forward_gflops": round(5 * n * math.log2(n) / (np.mean(fwd_ms) * 1e-3) / 1e9, 2),
+            "step_hbm_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "fft_lds_pass (forward)",
+            "bytes_per_launch": int(bytes_launch),
+            "avg_launch_ms": round(avg_launch_ms, 4),
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(x, dims, prec)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

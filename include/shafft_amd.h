@@ -1,0 +1,193 @@
+/* shafft_amd.h - C ABI of the gfx950 (MI355X) slab C2C FFT backend.
+ *
+ * Two groups of entry points, all extern "C", plain pointers and sizes:
+ *
+ *  1. The backend seam.  SHAFFT's core calls its FFT backend through a small
+ *     link-time symbol set (fftndPlan/fftndExecute/..., fft1d*, normalize*).
+ *     Each shafftAmd* function below replaces one of them; the C++ adapter
+ *     that binds them to the reference's FFTNDHandle / FFT1DHandle types is
+ *     shown in INTEGRATION.md.  Plans are opaque handles owned by the
+ *     library; data/work buffers are owned by the caller.
+ *
+ *  2. The reference's public C API (include/shafft/shafft.h) at one rank per
+ *     GPU: the same function names, argument order, enum values and status
+ *     codes.  The MPI communicator argument is kept in its position as an
+ *     opaque handle that must describe a one-rank communicator (pass 0); the
+ *     process grid is fixed to {1,...,1}.
+ *
+ * Directions: 0 = forward (exp(-2 pi i k.x / N)), 1 = backward, both
+ * unnormalised.  Precision: 0 = complex float (C2C), 1 = complex double (Z2Z).
+ * Tensors are row-major, last axis fastest.  Every status is 0 on success or
+ * a shafft_status_t value; the thread-local last-error record gives details.
+ */
+#ifndef SHAFFT_AMD_H
+#define SHAFFT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================== */
+/* 1. Backend seam                                                          */
+/* ======================================================================== */
+
+typedef struct shafft_amd_nd_plan shafft_amd_nd_plan;
+typedef struct shafft_amd_1d_plan shafft_amd_1d_plan;
+
+/* Replaces fftndPlan (src/cpp/fft/hipfft_method/fftnd_method.hpp:14-21).
+ * Local N-D transform over the consecutive axes ta[0..nta-1] of the
+ * row-major tensor dimensions[0..ndim-1].  in/out may be NULL (as in the
+ * reference: tests/unit/test_fft_partial.cpp:151-158). */
+int shafftAmdNDPlan(shafft_amd_nd_plan** plan, int nta, const int* ta, int ndim,
+                    const int* dimensions, int precision, void* in, void* out);
+
+/* Replaces fftndSetStream (fftnd_method.hpp:23).  stream is a hipStream_t. */
+int shafftAmdNDSetStream(shafft_amd_nd_plan* plan, void* stream);
+
+/* Replaces fftndExecute (fftnd_method.hpp:25): reads *data, may overwrite
+ * *work and *data, and on return *data holds the result (the two pointers
+ * are swapped when needed, as in fftnd_method.cpp:514-526).  Returns after
+ * the stream has drained (fftnd_method.cpp:520). */
+int shafftAmdNDExecute(shafft_amd_nd_plan* plan, void** data, void** work, int direction);
+
+/* Replaces fftndDestroy (fftnd_method.hpp:27). */
+int shafftAmdNDDestroy(shafft_amd_nd_plan* plan);
+
+/* Replaces fftndFinalize (fftnd_method.hpp:29-31). */
+int shafftAmdNDFinalize(void);
+
+/* Replace normalizeComplexFloat / normalizeComplexDouble
+ * (src/cpp/fft/hipfft_method/common/normalize.hpp): x[i] *= factor over
+ * tensor_size complex elements, asynchronous on stream. */
+int shafftAmdNormalizeComplexFloat(float factor, size_t tensor_size, void* data, void* stream);
+int shafftAmdNormalizeComplexDouble(double factor, size_t tensor_size, void* data, void* stream);
+
+/* Replaces shafft::fft1dQueryLayout (fft1d_method.hpp:34-41) for one rank. */
+int shafftAmd1DQueryLayout(size_t globalN, size_t* localN, size_t* localStart,
+                           size_t* localAllocSize, size_t* localNTrans,
+                           size_t* localStartTrans);
+
+/* Replaces shafft::fft1dPlan (fft1d_method.hpp:16-22) for one rank. */
+int shafftAmd1DPlan(shafft_amd_1d_plan** plan, size_t globalN, int precision, void* in,
+                    void* out);
+
+/* Replaces shafft::fft1dExecute (fft1d_method.hpp:24): in -> out, input
+ * preserved unless in == out.  Asynchronous on the plan's stream. */
+int shafftAmd1DExecute(shafft_amd_1d_plan* plan, void* in, void* out, int direction);
+
+/* Replace shafft::fft1dSetStream / fft1dSynchronize / fft1dDestroy. */
+int shafftAmd1DSetStream(shafft_amd_1d_plan* plan, void* stream);
+int shafftAmd1DSynchronize(shafft_amd_1d_plan* plan);
+int shafftAmd1DDestroy(shafft_amd_1d_plan* plan);
+
+/* Replaces shafft::fft1dNormalize (fft1d_method.hpp:32): factor
+ * 1/sqrt(N)^normExponent in the working precision, then synchronise. */
+int shafftAmd1DNormalize(shafft_amd_1d_plan* plan, void* data, int normExponent,
+                         size_t localCount);
+
+/* Introspection used by the benchmark and profiles (no reference
+ * counterpart): number of LDS passes, algorithmic HBM bytes of pass i, and a
+ * single-pass launch in -> out (asynchronous on the plan's stream). */
+int shafftAmdNDNumPasses(shafft_amd_nd_plan* plan, int* npasses);
+int shafftAmdNDPassBytes(shafft_amd_nd_plan* plan, int pass, int64_t* bytes);
+int shafftAmdNDExecutePass(shafft_amd_nd_plan* plan, int pass, void* in, void* out,
+                           int direction);
+int shafftAmd1DNumPasses(shafft_amd_1d_plan* plan, int* npasses);
+int shafftAmd1DPassBytes(shafft_amd_1d_plan* plan, int pass, int64_t* bytes);
+
+/* Human-readable pass list of a plan into buf (NUL-terminated, truncated). */
+int shafftAmdNDDescribe(shafft_amd_nd_plan* plan, char* buf, int buflen);
+int shafftAmd1DDescribe(shafft_amd_1d_plan* plan, char* buf, int buflen);
+
+/* ======================================================================== */
+/* 2. Reference C API (include/shafft/shafft.h) at one rank per GPU          */
+/* ======================================================================== */
+
+typedef enum { SHAFFT_C2C, SHAFFT_Z2Z } shafft_t;
+typedef enum { SHAFFT_FORWARD, SHAFFT_BACKWARD } shafft_direction_t;
+typedef enum {
+  SHAFFT_TENSOR_LAYOUT_CURRENT,
+  SHAFFT_TENSOR_LAYOUT_INITIAL,
+  SHAFFT_TENSOR_LAYOUT_REDISTRIBUTED
+} shafft_tensor_layout_t;
+typedef enum { SHAFFT_LAYOUT_REDISTRIBUTED = 0, SHAFFT_LAYOUT_INITIAL = 1 } shafft_transform_layout_t;
+typedef enum {
+  SHAFFT_SUCCESS = 0,
+  SHAFFT_ERR_NULLPTR = 1,
+  SHAFFT_ERR_INVALID_COMM = 2,
+  SHAFFT_ERR_NO_BUFFER = 3,
+  SHAFFT_ERR_PLAN_NOT_INIT = 4,
+  SHAFFT_ERR_INVALID_DIM = 5,
+  SHAFFT_ERR_DIM_MISMATCH = 6,
+  SHAFFT_ERR_INVALID_DECOMP = 7,
+  SHAFFT_ERR_INVALID_FFTTYPE = 8,
+  SHAFFT_ERR_ALLOC = 9,
+  SHAFFT_ERR_BACKEND = 10,
+  SHAFFT_ERR_MPI = 11,
+  SHAFFT_ERR_INVALID_LAYOUT = 12,
+  SHAFFT_ERR_SIZE_OVERFLOW = 13,
+  SHAFFT_ERR_NOT_IMPL = 14,
+  SHAFFT_ERR_INVALID_STATE = 15,
+  SHAFFT_ERR_INTERNAL = 16
+} shafft_status_t;
+typedef enum {
+  SHAFFT_ERRSRC_NONE = 0,
+  SHAFFT_ERRSRC_MPI = 1,
+  SHAFFT_ERRSRC_HIP = 2,
+  SHAFFT_ERRSRC_HIPFFT = 3,
+  SHAFFT_ERRSRC_FFTW = 4,
+  SHAFFT_ERRSRC_SYSTEM = 5
+} shafft_errsrc_t;
+
+/* Opaque stand-in for MPI_Comm at one rank (0 = the calling process). */
+typedef intptr_t shafft_comm_t;
+
+int shafftSetStream(void* planPtr, void* stream);                     /* shafft.h:42  */
+int shafftLastErrorStatus(void);                                      /* shafft.h:144 */
+int shafftLastErrorSource(void);                                      /* shafft.h:151 */
+int shafftLastErrorDomainCode(void);                                  /* shafft.h:166 */
+int shafftLastErrorMessage(char* buf, int buflen);                    /* shafft.h:175 */
+void shafftClearLastError(void);                                      /* shafft.h:183 */
+const char* shafftErrorSourceName(int source);                        /* shafft.h:191 */
+int shafftNDCreate(void** outPlan);                                   /* shafft.h:529 */
+int shafftNDInit(void* planPtr, int ndim, int commDims[], int dimensions[], shafft_t precision,
+                 shafft_comm_t cComm, shafft_transform_layout_t output_policy); /* :544 */
+int shafftPlan(void* planPtr);                                        /* shafft.h:562 */
+int shafftDestroy(void** planPtr);                                    /* shafft.h:573 */
+int shafftGetLayout(void* planPtr, size_t* subsize, size_t* offset,
+                    shafft_tensor_layout_t layout);                   /* shafft.h:585 */
+int shafftGetAxes(void* planPtr, int* ca, int* da, shafft_tensor_layout_t layout); /* :597 */
+int shafftGetAllocSize(void* plan, size_t* localAllocSize);           /* shafft.h:607 */
+int shafftGetGlobalSize(void* plan, size_t* globalSize);              /* shafft.h:617 */
+int shafftIsConfigured(void* plan, int* configured);                  /* shafft.h:627 */
+int shafftIsActive(void* plan, int* active);                          /* shafft.h:637 */
+int shafftSetBuffers(void* planPtr, void* data, void* work);          /* shafft.h:650 */
+int shafftGetBuffers(void* planPtr, void** data, void** work);        /* shafft.h:664 */
+int shafftExecute(void* planPtr, shafft_direction_t direction);       /* shafft.h:674 */
+int shafftNormalize(void* planPtr);                                   /* shafft.h:683 */
+int shafftConfiguration1D(size_t N, size_t* localN, size_t* localStart, size_t* localAllocSize,
+                          shafft_t precision, shafft_comm_t cComm);   /* shafft.h:697 */
+int shafft1DCreate(void** outPlan);                                   /* shafft.h:711 */
+int shafft1DInit(void* planPtr, size_t N, size_t localN, size_t localStart, shafft_t precision,
+                 shafft_comm_t cComm);                                /* shafft.h:725 */
+int shafftAllocBufferF(size_t count, void** buf);                     /* shafft.h:738 */
+int shafftAllocBufferD(size_t count, void** buf);                     /* shafft.h:750 */
+int shafftFreeBufferF(void* buf);                                     /* shafft.h:758 */
+int shafftFreeBufferD(void* buf);                                     /* shafft.h:766 */
+int shafftCopyToBufferF(void* dst, const void* src, size_t count);    /* shafft.h:777 */
+int shafftCopyToBufferD(void* dst, const void* src, size_t count);    /* shafft.h:788 */
+int shafftCopyFromBufferF(void* dst, const void* src, size_t count);  /* shafft.h:799 */
+int shafftCopyFromBufferD(void* dst, const void* src, size_t count);  /* shafft.h:810 */
+const char* shafftGetBackendName(void);                               /* shafft.h:817 */
+void shafftGetVersion(int* major, int* minor, int* patch);            /* shafft.h:827 */
+const char* shafftGetVersionString(void);                             /* shafft.h:834 */
+int shafftFinalize(void);                                             /* shafft.h:845 */
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* SHAFFT_AMD_H */

@@ -1,0 +1,116 @@
+#include "backend.hpp"
+
+#include <algorithm>
+#include <sstream>
+
+namespace sa {
+
+// Grow s1/s2 to what slot map `m` needs (allocation happens at plan time
+// for the primary mode and on first use for the aliased-buffer mode).
+static void ensure_scratch(const PassProgram::SlotMap& m, int prec, DevBuf& s1, DevBuf& s2) {
+  const size_t esz = prec ? 16 : 8;
+  if ((size_t)m.scratch[0] * esz > s1.bytes) s1 = DevBuf((size_t)m.scratch[0] * esz);
+  if ((size_t)m.scratch[1] * esz > s2.bytes) s2 = DevBuf((size_t)m.scratch[1] * esz);
+}
+
+NDBackend::NDBackend(int nta, const int* ta, int ndim, const int* dims, int prec)
+    : prec_(prec), prog_(prec) {
+  if (prec != 0 && prec != 1) fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
+  if (ndim < 1 || !dims) fail(ERR_INVALID_DIM, "invalid ndim");
+  for (int i = 0; i < ndim; ++i)
+    if (dims[i] <= 0) fail(ERR_INVALID_DIM, "dimension must be positive");
+  if (nta < 1 || nta > ndim || !ta) fail(ERR_INVALID_DIM, "invalid transform axes");
+  std::vector<int> axes(ta, ta + nta);
+  std::sort(axes.begin(), axes.end());
+  for (int i = 0; i < nta; ++i) {
+    if (axes[i] < 0 || axes[i] >= ndim) fail(ERR_INVALID_DIM, "transform axis out of range");
+    // Same restriction as the reference seam (fftnd_method.cpp:89-97).
+    if (i + 1 < nta && axes[i] + 1 != axes[i + 1])
+      fail(ERR_INVALID_DIM, "transform axes must be consecutive");
+  }
+  for (int i = 0; i < ndim; ++i) elems_ *= dims[i];
+  // Fastest axis first: the contiguous pass runs on the freshest data.
+  for (int j = nta - 1; j >= 0; --j) {
+    const int ax = axes[j];
+    int64_t outer = 1, inner = 1;
+    for (int i = 0; i < ax; ++i) outer *= dims[i];
+    for (int i = ax + 1; i < ndim; ++i) inner *= dims[i];
+    const int64_t n = dims[ax];
+    std::vector<Digit> lines = {{outer, n * inner, n * inner}, {inner, 1, 1}};
+    prog_.add_transform(n, lines, inner, inner);
+  }
+  map_pp_ = prog_.assign(PassProgram::PINGPONG, elems_);
+  map_ip_ = prog_.assign(PassProgram::INPLACE, elems_);
+  ensure_scratch(map_pp_, prec_, s1_, s2_);
+  if (diag_enabled()) {
+    std::ostringstream os;
+    os << "ndim=" << ndim << " nta=" << nta << " prec=" << (prec ? "Z2Z" : "C2C") << " passes=\n"
+       << prog_.describe();
+    diag("fftndPlan", os.str());
+  }
+}
+
+void NDBackend::execute(void*& data, void*& work, int dir) {
+  if (!data || !work) fail(ERR_NO_BUFFER, "data/work buffer not set");
+  const bool inplace = data == work;
+  const PassProgram::SlotMap& m = inplace ? map_ip_ : map_pp_;
+  if (inplace) ensure_scratch(m, prec_, s1_, s2_);
+  void* slots[SLOT_COUNT] = {data, work, s1_.p, s2_.p};
+  prog_.run(m, slots, dir, stream_);
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  if (!inplace && m.result == SLOT_WORK) std::swap(data, work);
+}
+
+void NDBackend::execute_pass(size_t i, void* in, void* out, int dir) {
+  if (i >= prog_.npasses()) fail(ERR_INVALID_DIM, "pass index out of range");
+  PassProgram::SlotMap m;
+  m.in.assign(prog_.npasses(), SLOT_DATA);
+  m.out.assign(prog_.npasses(), SLOT_WORK);
+  void* slots[SLOT_COUNT] = {in, out, s1_.p, s2_.p};
+  prog_.run_one(m, i, slots, dir, stream_);
+}
+
+OneDBackend::OneDBackend(int64_t n, int prec) : n_(n), prec_(prec), prog_(prec) {
+  if (prec != 0 && prec != 1) fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
+  if (n < 1) fail(ERR_INVALID_DIM, "invalid length");
+  prog_.add_transform(n, std::vector<Digit>{}, 1, 1);
+  map_pr_ = prog_.assign(PassProgram::PRESERVE, n_);
+  map_ip_ = prog_.assign(PassProgram::INPLACE, n_);
+  ensure_scratch(map_pr_, prec_, s1_, s2_);
+  if (diag_enabled()) {
+    std::ostringstream os;
+    os << "N=" << n << " prec=" << (prec ? "Z2Z" : "C2C") << " passes=\n" << prog_.describe();
+    diag("fft1dPlan", os.str());
+  }
+}
+
+void OneDBackend::execute(void* in, void* out, int dir) {
+  if (!in || !out) fail(ERR_NO_BUFFER, "buffer not set");
+  const bool inplace = in == out;
+  const PassProgram::SlotMap& m = inplace ? map_ip_ : map_pr_;
+  if (inplace) ensure_scratch(m, prec_, s1_, s2_);
+  if (prog_.npasses() == 0) {
+    if (!inplace)
+      hip_check(hipMemcpyAsync(out, in, (size_t)n_ * (prec_ ? 16 : 8), hipMemcpyDeviceToDevice,
+                               stream_),
+                "hipMemcpyAsync");
+    return;
+  }
+  void* slots[SLOT_COUNT] = {in, out, s1_.p, s2_.p};
+  prog_.run(m, slots, dir, stream_);
+}
+
+void OneDBackend::execute_pass(size_t i, void* in, void* out, int dir) {
+  if (i >= prog_.npasses()) fail(ERR_INVALID_DIM, "pass index out of range");
+  PassProgram::SlotMap m;
+  m.in.assign(prog_.npasses(), SLOT_DATA);
+  m.out.assign(prog_.npasses(), SLOT_WORK);
+  void* slots[SLOT_COUNT] = {in, out, s1_.p, s2_.p};
+  prog_.run_one(m, i, slots, dir, stream_);
+}
+
+void scale_async(void* data, double f, uint64_t count, int prec, hipStream_t stream) {
+  hip_check((hipError_t)sa_launch_scale(data, f, count, prec, stream), "normalize launch");
+}
+
+}  // namespace sa

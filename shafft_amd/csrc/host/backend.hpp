@@ -1,0 +1,66 @@
+// Backend objects behind the seam the reference core calls
+// (src/cpp/fft/hipfft_method/fftnd_method.hpp:14-31,
+//  src/cpp/fft/hipfft_method/fft1d_method.hpp:14-43,
+//  src/cpp/fft/hipfft_method/common/normalize.hpp).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "planner.hpp"
+
+namespace sa {
+
+// N-D local transform over a consecutive block of axes of a row-major
+// tensor (the reference's one-rank FFTND subplan, fftnd_method.cpp:162-351).
+class NDBackend {
+ public:
+  NDBackend(int nta, const int* ta, int ndim, const int* dims, int prec);
+  // Out-of-place data -> work; afterwards `data` holds the result (pointers
+  // swapped when the last pass wrote `work`).  Host-synchronous.
+  void execute(void*& data, void*& work, int dir);
+  void set_stream(hipStream_t s) { stream_ = s; }
+  hipStream_t stream() const { return stream_; }
+  int prec() const { return prec_; }
+  int64_t elems() const { return elems_; }
+  const PassProgram& program() const { return prog_; }
+  // Profiling aid: run pass i alone from `in` to `out` (asynchronous).
+  void execute_pass(size_t i, void* in, void* out, int dir);
+
+ private:
+  int prec_;
+  int64_t elems_ = 1;
+  PassProgram prog_;
+  PassProgram::SlotMap map_pp_, map_ip_;
+  DevBuf s1_, s2_;
+  hipStream_t stream_ = nullptr;
+};
+
+// 1-D transform of one local line at one rank (the reference's FFT1D
+// backend; fft1d_method.hip:416-650).  Input is preserved.
+class OneDBackend {
+ public:
+  OneDBackend(int64_t n, int prec);
+  void execute(void* in, void* out, int dir);
+  void set_stream(hipStream_t s) { stream_ = s; }
+  hipStream_t stream() const { return stream_; }
+  int prec() const { return prec_; }
+  int64_t n() const { return n_; }
+  const PassProgram& program() const { return prog_; }
+  void execute_pass(size_t i, void* in, void* out, int dir);
+
+ private:
+  int64_t n_;
+  int prec_;
+  PassProgram prog_;
+  PassProgram::SlotMap map_pr_, map_ip_;
+  DevBuf s1_, s2_;
+  hipStream_t stream_ = nullptr;
+};
+
+// x *= f over `count` complex elements on `stream` (asynchronous).
+void scale_async(void* data, double f, uint64_t count, int prec, hipStream_t stream);
+
+}  // namespace sa

@@ -1,0 +1,654 @@
+// C ABI: backend seam (shafftAmd*) and the reference's public C API at one
+// rank per GPU (shafft*).  The plan state machine, normalisation exponent
+// bookkeeping and buffer-swap contract follow the reference core:
+//   state machine      src/cpp/shafft.cpp:55-75,172-186,913-983
+//   configure checks   src/cpp/shafft_internal.cpp:229-333
+//   execute/normalize  src/cpp/shafft_internal.cpp:62-188 (ND),
+//                      src/cpp/shafft_internal.cpp:584-643 (1D)
+//   1-D factor         src/cpp/fft/hipfft_method/fft1d_method.hip:652-679
+#include <hip/hip_runtime_api.h>
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../../include/shafft_amd.h"
+#include "backend.hpp"
+#include "common.hpp"
+
+using namespace sa;
+
+#define SHAFFT_AMD_VERSION_MAJOR 1
+#define SHAFFT_AMD_VERSION_MINOR 1
+#define SHAFFT_AMD_VERSION_PATCH 0
+
+struct shafft_amd_nd_plan {
+  std::unique_ptr<NDBackend> be;
+};
+struct shafft_amd_1d_plan {
+  std::unique_ptr<OneDBackend> be;
+};
+
+namespace {
+
+constexpr uint32_t kMagic = 0x5AFF7A3Du;
+
+enum class Kind { ND, OneD };
+enum class State { UNINITIALIZED, CONFIGURED, PLANNED };
+
+// Plan object behind the reference C API's void* handles.
+struct Plan {
+  uint32_t magic = kMagic;
+  Kind kind;
+  State state = State::UNINITIALIZED;
+  int prec = 0;
+  std::vector<int> dims;  // ND extents (1D: {N})
+  size_t global = 0;
+  long double norm_denominator = 1.0L;
+  int norm_exponent = 0;
+  void* data = nullptr;
+  void* work = nullptr;
+  hipStream_t stream = nullptr;
+  int output_policy = 0;
+  int layout_state = 0;  // 1D: 0 INITIAL, 1 REDISTRIBUTED
+  std::unique_ptr<NDBackend> nd;
+  std::unique_ptr<OneDBackend> od;
+  explicit Plan(Kind k) : kind(k) {}
+};
+
+Plan* as_plan(void* p) {
+  if (!p) fail(ERR_NULLPTR, "null plan");
+  Plan* q = static_cast<Plan*>(p);
+  if (q->magic != kMagic) fail(ERR_NULLPTR, "not a plan handle");
+  return q;
+}
+
+void check_comm(shafft_comm_t comm) {
+  if (comm != 0) fail(ERR_INVALID_COMM, "only the one-rank communicator (0) is supported");
+}
+
+int prec_of(shafft_t t) {
+  if (t == SHAFFT_C2C) return 0;
+  if (t == SHAFFT_Z2Z) return 1;
+  fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
+}
+
+}  // namespace
+
+// =========================================================================
+// 1. Backend seam
+// =========================================================================
+extern "C" {
+
+int shafftAmdNDPlan(shafft_amd_nd_plan** plan, int nta, const int* ta, int ndim,
+                    const int* dimensions, int precision, void* in, void* out) {
+  (void)in;
+  (void)out;
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan pointer");
+    *plan = nullptr;
+    auto p = std::make_unique<shafft_amd_nd_plan>();
+    p->be = std::make_unique<NDBackend>(nta, ta, ndim, dimensions, precision);
+    *plan = p.release();
+    return 0;
+  });
+}
+
+int shafftAmdNDSetStream(shafft_amd_nd_plan* plan, void* stream) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    plan->be->set_stream(reinterpret_cast<hipStream_t>(stream));
+    return 0;
+  });
+}
+
+int shafftAmdNDExecute(shafft_amd_nd_plan* plan, void** data, void** work, int direction) {
+  return guarded([&] {
+    if (!plan || !data || !work) fail(ERR_NULLPTR, "null argument");
+    plan->be->execute(*data, *work, direction == 0 ? -1 : +1);
+    return 0;
+  });
+}
+
+int shafftAmdNDDestroy(shafft_amd_nd_plan* plan) {
+  return guarded([&] {
+    delete plan;
+    return 0;
+  });
+}
+
+int shafftAmdNDFinalize(void) { return 0; }
+
+int shafftAmdNormalizeComplexFloat(float factor, size_t tensor_size, void* data, void* stream) {
+  return guarded([&] {
+    scale_async(data, factor, tensor_size, 0, reinterpret_cast<hipStream_t>(stream));
+    return 0;
+  });
+}
+
+int shafftAmdNormalizeComplexDouble(double factor, size_t tensor_size, void* data, void* stream) {
+  return guarded([&] {
+    scale_async(data, factor, tensor_size, 1, reinterpret_cast<hipStream_t>(stream));
+    return 0;
+  });
+}
+
+int shafftAmd1DQueryLayout(size_t globalN, size_t* localN, size_t* localStart,
+                           size_t* localAllocSize, size_t* localNTrans,
+                           size_t* localStartTrans) {
+  return guarded([&] {
+    if (!localN || !localStart || !localAllocSize) fail(ERR_NULLPTR, "null output");
+    if (globalN == 0) fail(ERR_INVALID_DIM, "N must be positive");
+    // One rank: L = N, the whole sequence is local before and after.
+    *localN = globalN;
+    *localStart = 0;
+    *localAllocSize = globalN;
+    if (localNTrans) *localNTrans = globalN;
+    if (localStartTrans) *localStartTrans = 0;
+    return 0;
+  });
+}
+
+int shafftAmd1DPlan(shafft_amd_1d_plan** plan, size_t globalN, int precision, void* in,
+                    void* out) {
+  (void)in;
+  (void)out;
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan pointer");
+    *plan = nullptr;
+    if (globalN == 0) fail(ERR_INVALID_DIM, "N must be positive");
+    auto p = std::make_unique<shafft_amd_1d_plan>();
+    p->be = std::make_unique<OneDBackend>((int64_t)globalN, precision);
+    *plan = p.release();
+    return 0;
+  });
+}
+
+int shafftAmd1DExecute(shafft_amd_1d_plan* plan, void* in, void* out, int direction) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    plan->be->execute(in, out, direction == 0 ? -1 : +1);
+    return 0;
+  });
+}
+
+int shafftAmd1DSetStream(shafft_amd_1d_plan* plan, void* stream) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    plan->be->set_stream(reinterpret_cast<hipStream_t>(stream));
+    return 0;
+  });
+}
+
+int shafftAmd1DSynchronize(shafft_amd_1d_plan* plan) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    hip_check(hipStreamSynchronize(plan->be->stream()), "hipStreamSynchronize");
+    return 0;
+  });
+}
+
+int shafftAmd1DDestroy(shafft_amd_1d_plan* plan) {
+  return guarded([&] {
+    delete plan;
+    return 0;
+  });
+}
+
+int shafftAmd1DNormalize(shafft_amd_1d_plan* plan, void* data, int normExponent,
+                         size_t localCount) {
+  return guarded([&] {
+    if (!plan) fail(ERR_PLAN_NOT_INIT, "null plan");
+    const size_t n = (size_t)plan->be->n();
+    double f;
+    if (plan->be->prec() == 0) {
+      // Working precision, as fft1d_method.hip:663-668.
+      float norm = 1.0f / std::pow(std::sqrt(static_cast<float>(n)), normExponent);
+      f = norm;
+    } else {
+      f = 1.0 / std::pow(std::sqrt(static_cast<double>(n)), normExponent);
+    }
+    scale_async(data, f, localCount, plan->be->prec(), plan->be->stream());
+    hip_check(hipStreamSynchronize(plan->be->stream()), "hipStreamSynchronize");
+    return 0;
+  });
+}
+
+int shafftAmdNDNumPasses(shafft_amd_nd_plan* plan, int* npasses) {
+  return guarded([&] {
+    if (!plan || !npasses) fail(ERR_NULLPTR, "null argument");
+    *npasses = (int)plan->be->program().npasses();
+    return 0;
+  });
+}
+
+int shafftAmdNDPassBytes(shafft_amd_nd_plan* plan, int pass, int64_t* bytes) {
+  return guarded([&] {
+    if (!plan || !bytes) fail(ERR_NULLPTR, "null argument");
+    if (pass < 0 || (size_t)pass >= plan->be->program().npasses())
+      fail(ERR_INVALID_DIM, "pass index out of range");
+    *bytes = plan->be->program().pass((size_t)pass).bytes;
+    return 0;
+  });
+}
+
+int shafftAmdNDExecutePass(shafft_amd_nd_plan* plan, int pass, void* in, void* out,
+                           int direction) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    if (pass < 0) fail(ERR_INVALID_DIM, "pass index out of range");
+    plan->be->execute_pass((size_t)pass, in, out, direction == 0 ? -1 : +1);
+    return 0;
+  });
+}
+
+int shafftAmd1DNumPasses(shafft_amd_1d_plan* plan, int* npasses) {
+  return guarded([&] {
+    if (!plan || !npasses) fail(ERR_NULLPTR, "null argument");
+    *npasses = (int)plan->be->program().npasses();
+    return 0;
+  });
+}
+
+int shafftAmd1DPassBytes(shafft_amd_1d_plan* plan, int pass, int64_t* bytes) {
+  return guarded([&] {
+    if (!plan || !bytes) fail(ERR_NULLPTR, "null argument");
+    if (pass < 0 || (size_t)pass >= plan->be->program().npasses())
+      fail(ERR_INVALID_DIM, "pass index out of range");
+    *bytes = plan->be->program().pass((size_t)pass).bytes;
+    return 0;
+  });
+}
+
+static int copy_desc(const std::string& s, char* buf, int buflen) {
+  if (!buf || buflen <= 0) fail(ERR_NULLPTR, "null buffer");
+  const size_t n = std::min<size_t>(s.size(), (size_t)buflen - 1);
+  std::memcpy(buf, s.data(), n);
+  buf[n] = 0;
+  return 0;
+}
+
+int shafftAmdNDDescribe(shafft_amd_nd_plan* plan, char* buf, int buflen) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    return copy_desc(plan->be->program().describe(), buf, buflen);
+  });
+}
+
+int shafftAmd1DDescribe(shafft_amd_1d_plan* plan, char* buf, int buflen) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    return copy_desc(plan->be->program().describe(), buf, buflen);
+  });
+}
+
+// =========================================================================
+// 2. Reference C API at one rank
+// =========================================================================
+
+int shafftLastErrorStatus(void) { return last_error_status(); }
+int shafftLastErrorSource(void) { return last_error_source(); }
+int shafftLastErrorDomainCode(void) { return last_error_domain(); }
+int shafftLastErrorMessage(char* buf, int buflen) {
+  if (!buf || buflen <= 0) return (int)ERR_NULLPTR;
+  const std::string& m = last_error_message();
+  const size_t n = std::min<size_t>(m.size(), (size_t)buflen - 1);
+  std::memcpy(buf, m.data(), n);
+  buf[n] = 0;
+  return (int)m.size();
+}
+void shafftClearLastError(void) { clear_last_error(); }
+const char* shafftErrorSourceName(int source) {
+  switch (source) {
+    case SHAFFT_ERRSRC_NONE: return "NONE";
+    case SHAFFT_ERRSRC_MPI: return "MPI";
+    case SHAFFT_ERRSRC_HIP: return "HIP";
+    case SHAFFT_ERRSRC_HIPFFT: return "hipFFT";
+    case SHAFFT_ERRSRC_FFTW: return "FFTW";
+    case SHAFFT_ERRSRC_SYSTEM: return "SYSTEM";
+    default: return "UNKNOWN";
+  }
+}
+
+int shafftNDCreate(void** outPlan) {
+  return guarded([&] {
+    if (!outPlan) fail(ERR_NULLPTR, "null outPlan");
+    *outPlan = new Plan(Kind::ND);
+    return 0;
+  });
+}
+
+int shafft1DCreate(void** outPlan) {
+  return guarded([&] {
+    if (!outPlan) fail(ERR_NULLPTR, "null outPlan");
+    *outPlan = new Plan(Kind::OneD);
+    return 0;
+  });
+}
+
+int shafftNDInit(void* planPtr, int ndim, int commDims[], int dimensions[], shafft_t precision,
+                 shafft_comm_t cComm, shafft_transform_layout_t output_policy) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    if (p->kind != Kind::ND) fail(ERR_INVALID_STATE, "not an N-D plan");
+    if (p->state != State::UNINITIALIZED) fail(ERR_INVALID_STATE, "plan already initialised");
+    const int prec = prec_of(precision);
+    if (output_policy != SHAFFT_LAYOUT_REDISTRIBUTED && output_policy != SHAFFT_LAYOUT_INITIAL)
+      fail(ERR_INVALID_LAYOUT, "invalid output policy");
+    check_comm(cComm);
+    if (!commDims || !dimensions) fail(ERR_NULLPTR, "null dims");
+    if (ndim <= 0) fail(ERR_INVALID_DIM, "ndim must be positive");
+    for (int i = 0; i < ndim; ++i)
+      if (dimensions[i] <= 0) fail(ERR_INVALID_DIM, "dimension must be positive");
+    if (commDims[ndim - 1] != 1) fail(ERR_INVALID_DECOMP, "last axis cannot be distributed");
+    for (int i = 0; i < ndim; ++i) {
+      if (commDims[i] < 1 || commDims[i] > dimensions[i])
+        fail(ERR_INVALID_DECOMP, "invalid process grid");
+      // One rank per plan: every grid extent must be 1.
+      if (commDims[i] != 1) fail(ERR_INVALID_DECOMP, "process grid exceeds the communicator");
+    }
+    long double prod = 1.0L;
+    size_t g = 1;
+    for (int i = 0; i < ndim; ++i) {
+      prod *= (long double)dimensions[i];
+      g *= (size_t)dimensions[i];
+    }
+    p->prec = prec;
+    p->dims.assign(dimensions, dimensions + ndim);
+    p->global = g;
+    p->norm_denominator = std::sqrt(prod);  // shafft_internal.cpp:329
+    p->norm_exponent = 0;
+    p->output_policy = output_policy;
+    p->state = State::CONFIGURED;
+    return 0;
+  });
+}
+
+int shafft1DInit(void* planPtr, size_t N, size_t localN, size_t localStart, shafft_t precision,
+                 shafft_comm_t cComm) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    if (p->kind != Kind::OneD) fail(ERR_INVALID_STATE, "not a 1-D plan");
+    if (p->state != State::UNINITIALIZED) fail(ERR_INVALID_STATE, "plan already initialised");
+    const int prec = prec_of(precision);
+    check_comm(cComm);
+    if (N == 0) fail(ERR_INVALID_DIM, "N must be positive");
+    if (localN != N || localStart != 0)
+      fail(ERR_INVALID_LAYOUT, "layout does not match the one-rank distribution");
+    p->prec = prec;
+    p->dims = {(int)std::min<size_t>(N, INT_MAX)};
+    p->global = N;
+    p->norm_exponent = 0;
+    p->layout_state = 0;
+    p->state = State::CONFIGURED;
+    return 0;
+  });
+}
+
+int shafftConfiguration1D(size_t N, size_t* localN, size_t* localStart, size_t* localAllocSize,
+                          shafft_t precision, shafft_comm_t cComm) {
+  return guarded([&] {
+    (void)prec_of(precision);
+    check_comm(cComm);
+    return shafftAmd1DQueryLayout(N, localN, localStart, localAllocSize, nullptr, nullptr);
+  });
+}
+
+int shafftPlan(void* planPtr) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    if (p->state == State::PLANNED) fail(ERR_INVALID_STATE, "plan() called twice");
+    if (p->state != State::CONFIGURED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
+    if (p->kind == Kind::ND) {
+      const int ndim = (int)p->dims.size();
+      std::vector<int> ta(ndim);
+      for (int i = 0; i < ndim; ++i) ta[i] = i;
+      p->nd = std::make_unique<NDBackend>(ndim, ta.data(), ndim, p->dims.data(), p->prec);
+      p->nd->set_stream(p->stream);
+    } else {
+      p->od = std::make_unique<OneDBackend>((int64_t)p->global, p->prec);
+      p->od->set_stream(p->stream);
+    }
+    p->state = State::PLANNED;
+    return 0;
+  });
+}
+
+int shafftDestroy(void** planPtr) {
+  return guarded([&] {
+    if (!planPtr || !*planPtr) fail(ERR_NULLPTR, "null plan");
+    Plan* p = as_plan(*planPtr);
+    p->magic = 0;
+    delete p;
+    *planPtr = nullptr;
+    return 0;
+  });
+}
+
+int shafftSetStream(void* planPtr, void* stream) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    // N-D: the stream is applied to backend subplans, which exist only
+    // after plan() (shafft_internal.cpp:198-210).
+    if (p->kind == Kind::ND && p->state != State::PLANNED)
+      fail(ERR_PLAN_NOT_INIT, "setStream before plan()");
+    p->stream = reinterpret_cast<hipStream_t>(stream);
+    if (p->nd) p->nd->set_stream(p->stream);
+    if (p->od) p->od->set_stream(p->stream);
+    return 0;
+  });
+}
+
+int shafftGetLayout(void* planPtr, size_t* subsize, size_t* offset,
+                    shafft_tensor_layout_t layout) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    if (p->state == State::UNINITIALIZED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
+    if (!subsize || !offset) fail(ERR_NULLPTR, "null output");
+    if (layout < SHAFFT_TENSOR_LAYOUT_CURRENT || layout > SHAFFT_TENSOR_LAYOUT_REDISTRIBUTED)
+      fail(ERR_INVALID_LAYOUT, "invalid layout");
+    if (p->kind == Kind::ND) {
+      for (size_t i = 0; i < p->dims.size(); ++i) {
+        subsize[i] = (size_t)p->dims[i];
+        offset[i] = 0;
+      }
+    } else {
+      subsize[0] = p->global;
+      offset[0] = 0;
+    }
+    return 0;
+  });
+}
+
+int shafftGetAxes(void* planPtr, int* ca, int* da, shafft_tensor_layout_t layout) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    if (p->state == State::UNINITIALIZED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
+    (void)da;
+    if (layout < SHAFFT_TENSOR_LAYOUT_CURRENT || layout > SHAFFT_TENSOR_LAYOUT_REDISTRIBUTED)
+      fail(ERR_INVALID_LAYOUT, "invalid layout");
+    if (p->kind == Kind::ND) {
+      if (!ca) fail(ERR_NULLPTR, "null ca");
+      // One rank: every axis is contiguous, none distributed (slab_nd.cpp:522-526).
+      for (size_t i = 0; i < p->dims.size(); ++i) ca[i] = (int)i;
+    }
+    return 0;
+  });
+}
+
+int shafftGetAllocSize(void* plan, size_t* localAllocSize) {
+  return guarded([&] {
+    Plan* p = as_plan(plan);
+    if (!localAllocSize) fail(ERR_NULLPTR, "null output");
+    if (p->state == State::UNINITIALIZED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
+    *localAllocSize = p->global;
+    return 0;
+  });
+}
+
+int shafftGetGlobalSize(void* plan, size_t* globalSize) {
+  return guarded([&] {
+    Plan* p = as_plan(plan);
+    if (!globalSize) fail(ERR_NULLPTR, "null output");
+    if (p->state == State::UNINITIALIZED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
+    *globalSize = p->global;
+    return 0;
+  });
+}
+
+int shafftIsConfigured(void* plan, int* configured) {
+  return guarded([&] {
+    if (!configured) fail(ERR_NULLPTR, "null output");
+    *configured = 0;
+    if (plan) *configured = as_plan(plan)->state != State::UNINITIALIZED ? 1 : 0;
+    return 0;
+  });
+}
+
+int shafftIsActive(void* plan, int* active) {
+  return guarded([&] {
+    if (!active) fail(ERR_NULLPTR, "null output");
+    *active = 0;
+    if (plan) *active = as_plan(plan)->state != State::UNINITIALIZED ? 1 : 0;
+    return 0;
+  });
+}
+
+int shafftSetBuffers(void* planPtr, void* data, void* work) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    if (p->state == State::UNINITIALIZED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
+    if (!data || !work) fail(ERR_NULLPTR, "null buffer");
+    p->data = data;
+    p->work = work;
+    return 0;
+  });
+}
+
+int shafftGetBuffers(void* planPtr, void** data, void** work) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    if (!data || !work) fail(ERR_NULLPTR, "null output");
+    if (p->state == State::UNINITIALIZED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
+    *data = p->data;
+    *work = p->work;
+    return 0;
+  });
+}
+
+int shafftExecute(void* planPtr, shafft_direction_t direction) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    if (p->state != State::PLANNED) fail(ERR_PLAN_NOT_INIT, "plan() not called");
+    if (direction != SHAFFT_FORWARD && direction != SHAFFT_BACKWARD)
+      fail(ERR_INVALID_STATE, "invalid direction");
+    if (!p->data || !p->work) fail(ERR_NO_BUFFER, "buffers not set");
+    const int dir = direction == SHAFFT_FORWARD ? -1 : +1;
+    if (p->kind == Kind::ND) {
+      p->nd->execute(p->data, p->work, dir);
+    } else {
+      // data -> work, then swap so data holds the result
+      // (shafft_internal.cpp:596-611); host-synchronous like the N-D path.
+      p->od->execute(p->data, p->work, dir);
+      hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+      if (p->work != p->data) std::swap(p->data, p->work);
+      p->layout_state = direction == SHAFFT_FORWARD ? 1 : 0;
+    }
+    p->norm_exponent += 1;
+    return 0;
+  });
+}
+
+int shafftNormalize(void* planPtr) {
+  return guarded([&] {
+    Plan* p = as_plan(planPtr);
+    if (p->state == State::UNINITIALIZED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
+    if (!p->data || !p->work) fail(ERR_NO_BUFFER, "buffers not set");
+    if (p->kind == Kind::ND) {
+      // factor = 1 / denominator^e in long double, rounded once to the
+      // working precision (shafft_internal.cpp:136,150-156).
+      long double f = 1.0 / std::pow(p->norm_denominator, p->norm_exponent);
+      const double fd = p->prec ? (double)f : (double)(float)f;
+      scale_async(p->data, fd, p->global, p->prec, p->stream);
+      hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    } else {
+      if (!p->od) fail(ERR_PLAN_NOT_INIT, "plan() not called");
+      const size_t n = p->global;
+      double f;
+      if (p->prec == 0) {
+        float norm = 1.0f / std::pow(std::sqrt(static_cast<float>(n)), p->norm_exponent);
+        f = norm;
+      } else {
+        f = 1.0 / std::pow(std::sqrt(static_cast<double>(n)), p->norm_exponent);
+      }
+      scale_async(p->data, f, n, p->prec, p->stream);
+      hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    }
+    p->norm_exponent = 0;
+    return 0;
+  });
+}
+
+int shafftAllocBufferF(size_t count, void** buf) {
+  return guarded([&] {
+    if (!buf) fail(ERR_NULLPTR, "null output");
+    *buf = nullptr;
+    hip_check(hipMalloc(buf, count * 8), "hipMalloc");
+    return 0;
+  });
+}
+
+int shafftAllocBufferD(size_t count, void** buf) {
+  return guarded([&] {
+    if (!buf) fail(ERR_NULLPTR, "null output");
+    *buf = nullptr;
+    hip_check(hipMalloc(buf, count * 16), "hipMalloc");
+    return 0;
+  });
+}
+
+int shafftFreeBufferF(void* buf) {
+  return guarded([&] {
+    if (buf) hip_check(hipFree(buf), "hipFree");
+    return 0;
+  });
+}
+
+int shafftFreeBufferD(void* buf) { return shafftFreeBufferF(buf); }
+
+static int copy_bytes(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  return guarded([&] {
+    if (!dst || !src) fail(ERR_NULLPTR, "null pointer");
+    hip_check(hipMemcpy(dst, src, bytes, kind), "hipMemcpy");
+    return 0;
+  });
+}
+
+int shafftCopyToBufferF(void* dst, const void* src, size_t count) {
+  return copy_bytes(dst, src, count * 8, hipMemcpyHostToDevice);
+}
+int shafftCopyToBufferD(void* dst, const void* src, size_t count) {
+  return copy_bytes(dst, src, count * 16, hipMemcpyHostToDevice);
+}
+int shafftCopyFromBufferF(void* dst, const void* src, size_t count) {
+  return copy_bytes(dst, src, count * 8, hipMemcpyDeviceToHost);
+}
+int shafftCopyFromBufferD(void* dst, const void* src, size_t count) {
+  return copy_bytes(dst, src, count * 16, hipMemcpyDeviceToHost);
+}
+
+const char* shafftGetBackendName(void) { return "gfx950"; }
+
+void shafftGetVersion(int* major, int* minor, int* patch) {
+  if (major) *major = SHAFFT_AMD_VERSION_MAJOR;
+  if (minor) *minor = SHAFFT_AMD_VERSION_MINOR;
+  if (patch) *patch = SHAFFT_AMD_VERSION_PATCH;
+}
+
+const char* shafftGetVersionString(void) { return "1.1.0"; }
+
+int shafftFinalize(void) { return 0; }
+
+}  // extern "C"

@@ -1,0 +1,590 @@
+#include "planner.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+
+#include "fastdiv.hpp"
+#include "tables.hpp"
+
+namespace sa {
+
+// ---------------------------------------------------------------------------
+DevBuf::DevBuf(size_t n) : bytes(n) {
+  if (n) hip_check(hipMalloc(&p, n), "hipMalloc(table/scratch)");
+}
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+  if (this != &o) {
+    if (p) (void)hipFree(p);
+    p = o.p;
+    bytes = o.bytes;
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  return *this;
+}
+
+// ---------------------------------------------------------------------------
+static int lds_budget_bytes() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 0;
+    cached = 64 * 1024;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 0)
+      cached = std::min(v, sa_max_lds_bytes());
+  }
+  return cached;
+}
+
+// Lines up to this length run as one LDS-resident pass.  A line must fit
+// the LDS of one workgroup with room for a second one per CU.
+// SHAFFT_AMD_LDS_CAP (elements, >= 8) lowers the cap: used to exercise the
+// multi-level four-step and Bluestein plans on small inputs.
+int64_t lds_cap(int prec) {
+  const int64_t hw = prec ? 4096 : 8192;
+  if (const char* e = std::getenv("SHAFFT_AMD_LDS_CAP")) {
+    const long long v = std::atoll(e);
+    if (v >= 8 && v < hw) return v;
+  }
+  return hw;
+}
+
+bool is_smooth7(int64_t n) {
+  if (n < 1) return false;
+  for (int p : {2, 3, 5, 7})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+std::vector<int> radix_plan(int64_t n) {
+  std::vector<int> r;
+  int a = 0;
+  while (n % 2 == 0) {
+    n /= 2;
+    ++a;
+  }
+  // Powers of two: radix-16 stages, the remainder as one radix 8/4/2 stage;
+  // 2^(4q+1) is taken as 16^(q-1) * 8 * 4 rather than 16^q * 2.
+  int q = a / 4, rem = a % 4;
+  if (rem == 1 && q >= 1) {
+    for (int i = 0; i < q - 1; ++i) r.push_back(16);
+    r.push_back(8);
+    r.push_back(4);
+  } else {
+    for (int i = 0; i < q; ++i) r.push_back(16);
+    if (rem == 3) r.push_back(8);
+    if (rem == 2) r.push_back(4);
+    if (rem == 1) r.push_back(2);
+  }
+  for (int p : {3, 5, 7})
+    while (n % p == 0) {
+      n /= p;
+      r.push_back(p);
+    }
+  if (n != 1) fail(ERR_INTERNAL, "radix_plan: length is not 7-smooth");
+  return r;
+}
+
+static int64_t next_pow2(int64_t x) {
+  int64_t m = 1;
+  while (m < x) m <<= 1;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+const void* PassProgram::upload(const std::vector<cld>& t) {
+  const size_t n = t.size();
+  DevBuf b(n * (prec_ ? 16 : 8));
+  if (prec_) {
+    std::vector<double> h(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+      h[2 * i] = (double)t[i].real();
+      h[2 * i + 1] = (double)t[i].imag();
+    }
+    hip_check(hipMemcpy(b.p, h.data(), b.bytes, hipMemcpyHostToDevice), "hipMemcpy(table)");
+  } else {
+    std::vector<float> h(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+      h[2 * i] = (float)t[i].real();
+      h[2 * i + 1] = (float)t[i].imag();
+    }
+    hip_check(hipMemcpy(b.p, h.data(), b.bytes, hipMemcpyHostToDevice), "hipMemcpy(table)");
+  }
+  const void* p = b.p;
+  tables_.push_back(std::move(b));
+  return p;
+}
+
+const void* PassProgram::stage_table(const std::vector<int>& radices, std::vector<int>& offsets) {
+  return upload(stage_twiddles(radices, offsets));
+}
+
+// Merge adjacent digits that address memory as one digit on both sides and
+// drop unit digits, never touching the digits listed in `pinned`.
+static std::vector<Digit> merge_digits(const std::vector<Digit>& in, std::vector<int>& pinned) {
+  std::vector<Digit> d;
+  std::vector<int> map(in.size(), -1);
+  for (size_t j = 0; j < in.size(); ++j) {
+    const bool pin = std::find(pinned.begin(), pinned.end(), (int)j) != pinned.end();
+    if (in[j].cnt == 1 && !pin) continue;
+    if (!d.empty() && !pin) {
+      // previous kept digit
+      int prev = -1;
+      for (int i = (int)j - 1; i >= 0; --i)
+        if (map[i] >= 0) {
+          prev = i;
+          break;
+        }
+      const bool prev_pin =
+          prev >= 0 && std::find(pinned.begin(), pinned.end(), prev) != pinned.end();
+      Digit& o = d.back();
+      if (!prev_pin && o.is == in[j].cnt * in[j].is && o.os == in[j].cnt * in[j].os) {
+        o.cnt *= in[j].cnt;
+        o.is = in[j].is;
+        o.os = in[j].os;
+        map[j] = (int)d.size() - 1;
+        continue;
+      }
+    }
+    d.push_back(in[j]);
+    map[j] = (int)d.size() - 1;
+  }
+  for (int& p : pinned) p = map[p];
+  return d;
+}
+
+// Fill launch shape: threads per line, lines per workgroup, lane mappings.
+static void configure_launch(sa_pass_t& p, int elem_bytes) {
+  const int n = p.n;
+  const int emax = 16;
+  int T = (n + emax - 1) / emax;
+  if (T < 1) T = 1;
+  if (T > 512) fail(ERR_INTERNAL, "pass length exceeds the per-workgroup thread budget");
+  const bool ld_lf = p.iks != 1 && p.is2 == 1 && p.n2 > 1;
+  const bool st_lf = p.oks != 1 && p.os2 == 1 && p.n2 > 1;
+  p.ld_kfast = ld_lf ? 0 : 1;
+  p.st_kfast = st_lf ? 0 : 1;
+  const bool lf = ld_lf || st_lf;
+  int C = std::max(1, 256 / T);
+  if (lf) C = std::max(C, elem_bytes == 8 ? 16 : 8);
+  // line-fast lanes walk lines: keep C within the fast digit
+  if (lf) C = (int)std::min<int64_t>(C, p.n2);
+  while (C > 1 && C * T > 512) C /= 2;
+  int stride = n;
+  if (lf && (n % 2 == 0)) stride = n + 1;
+  const int budget = lds_budget_bytes();
+  while (C > 1 && (int64_t)C * stride * elem_bytes > budget) C /= 2;
+  if ((int64_t)C * stride * elem_bytes > budget) fail(ERR_INTERNAL, "pass does not fit LDS");
+  if (C > p.batch) C = (int)std::max<int64_t>(1, p.batch);
+  p.T = T;
+  p.C = C;
+  p.ldsstride = stride;
+  p.Tdiv = make_fastdiv((uint32_t)T);
+  p.Cdiv = make_fastdiv((uint32_t)C);
+}
+
+void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, int64_t oks,
+                              int sign, const ElemHook& ld, const ElemHook& st, const void* blu_h,
+                              int ptw_level, int64_t ptw_N, int64_t out_elems) {
+  // Digits referenced by hooks or the twiddle keep their identity through
+  // merging; order of `lv` below is the order they are remapped in.
+  int lv[5] = {ld.level, ld.level2, st.level, st.level2, ptw_level};
+  std::vector<int> pin_in;
+  for (int v : lv)
+    if (v >= 0) pin_in.push_back(v);
+  std::vector<Digit> d = merge_digits(lines, pin_in);
+  if (d.size() > 3) fail(ERR_NOT_IMPL, "pass needs more than three line digits");
+  {
+    int idx = 0;
+    for (int& v : lv)
+      if (v >= 0) v = pin_in[idx++];
+  }
+  int& ld_level = lv[0];
+  int& ld_level2 = lv[1];
+  int& st_level = lv[2];
+  int& st_level2 = lv[3];
+  int& pt_level = lv[4];
+  // The lane-fast digit of a line-fast side must be the last one (b2):
+  // move a unit-stride digit there when the element stride is not unit.
+  {
+    int cand = -1;
+    for (int j = 0; j < (int)d.size(); ++j)
+      if (iks != 1 && oks != 1 && d[j].is == 1 && d[j].os == 1) cand = j;
+    for (int j = 0; cand < 0 && j < (int)d.size(); ++j)
+      if (oks != 1 && d[j].os == 1) cand = j;
+    for (int j = 0; cand < 0 && j < (int)d.size(); ++j)
+      if (iks != 1 && d[j].is == 1) cand = j;
+    const int last = (int)d.size() - 1;
+    if (cand >= 0 && cand != last) {
+      Digit mv = d[cand];
+      d.erase(d.begin() + cand);
+      d.push_back(mv);
+      auto remap = [&](int& lv) {
+        if (lv < 0) return;
+        if (lv == cand) lv = last;
+        else if (lv > cand) --lv;
+      };
+      for (int& v : lv) remap(v);
+    }
+  }
+  // left-pad to three digits
+  const int pad = 3 - (int)d.size();
+  while (d.size() < 3) d.insert(d.begin(), Digit{1, 0, 0});
+  for (int& v : lv)
+    if (v >= 0) v += pad;
+
+  PassSpec ps;
+  sa_pass_t& p = ps.p;
+  std::memset(&p, 0, sizeof(p));
+  int64_t batch = d[0].cnt * d[1].cnt * d[2].cnt;
+  if (batch >= (int64_t(1) << 31) || d[1].cnt >= (int64_t(1) << 31) ||
+      d[2].cnt >= (int64_t(1) << 31))
+    fail(ERR_SIZE_OVERFLOW, "pass line count exceeds 2^31");
+  p.batch = batch;
+  p.n1 = d[1].cnt;
+  p.n2 = d[2].cnt;
+  p.is0 = d[0].is;
+  p.is1 = d[1].is;
+  p.is2 = d[2].is;
+  p.os0 = d[0].os;
+  p.os1 = d[1].os;
+  p.os2 = d[2].os;
+  p.iks = iks;
+  p.oks = oks;
+  p.n = (int32_t)N;
+  p.n1div = make_fastdiv((uint32_t)p.n1);
+  p.n2div = make_fastdiv((uint32_t)p.n2);
+
+  std::vector<int> rad = N > 1 ? radix_plan(N) : std::vector<int>{};
+  if (rad.size() > SA_MAX_STAGES) fail(ERR_INTERNAL, "too many stages");
+  std::vector<int> offs;
+  p.tw = stage_table(rad, offs);
+  p.nstages = (int32_t)rad.size();
+  int64_t Ns = 1;
+  for (size_t s = 0; s < rad.size(); ++s) {
+    p.radix[s] = rad[s];
+    p.ns[s] = (int32_t)Ns;
+    p.twoff[s] = offs[s];
+    p.nsdiv[s] = make_fastdiv((uint32_t)Ns);
+    Ns *= rad[s];
+  }
+  p.ptw_level = pt_level;
+  if (pt_level >= 0) {
+    std::vector<cld> lo, hi;
+    int shift = 0;
+    two_level(ptw_N, shift, lo, hi);
+    p.ptw_shift = shift;
+    p.ptw_N = ptw_N;
+    p.ptw_lo = upload(lo);
+    p.ptw_hi = upload(hi);
+  }
+  p.ld_kmul = ld.kmul;
+  p.ld_dmul = ld.dmul;
+  p.ld_level = ld_level;
+  p.ld_dmul2 = ld.dmul2;
+  p.ld_level2 = ld_level2;
+  p.st_dmul2 = st.dmul2;
+  p.st_level2 = st_level2;
+  p.ld_limit = ld.limit >= 0 ? ld.limit : INT64_MAX;
+  p.st_kmul = st.kmul;
+  p.st_dmul = st.dmul;
+  p.st_level = st_level;
+  p.st_limit = st.limit >= 0 ? st.limit : INT64_MAX;
+  p.pre = ld.table;
+  p.post = st.table;
+  p.blu_h = blu_h;
+  p.scale = 1.0;
+  configure_launch(p, prec_ ? 16 : 8);
+
+  ps.sign = sign;
+  ps.out_elems = out_elems;
+  const int64_t esz = prec_ ? 16 : 8;
+  const int64_t rd = batch * std::min<int64_t>(N, p.ld_limit < N ? p.ld_limit : N);
+  const int64_t wr = batch * std::min<int64_t>(N, p.st_limit < N ? p.st_limit : N);
+  ps.bytes = (rd + wr) * esz;
+  passes_.push_back(ps);
+}
+
+static int64_t prod_cnt(const std::vector<Digit>& d) {
+  int64_t p = 1;
+  for (auto& x : d) p *= x.cnt;
+  return p;
+}
+
+// Canonical contiguous layout for the line digits of an intermediate whose
+// lines have `len` elements: digit j stride = len * prod(cnt_{>j}).
+static std::vector<int64_t> canon_strides(const std::vector<Digit>& d, int64_t len) {
+  std::vector<int64_t> s(d.size());
+  int64_t acc = len;
+  for (int j = (int)d.size() - 1; j >= 0; --j) {
+    s[j] = acc;
+    acc *= d[j].cnt;
+  }
+  return s;
+}
+
+static int64_t choose_n1(int64_t N, int64_t cap) {
+  // Number of levels so that every factor fits; then the largest smooth
+  // divisor not above the balanced size N^(1/levels).
+  int levels = 2;
+  while (true) {
+    long double f = powl((long double)N, 1.0L / levels);
+    if (f <= (long double)cap * 1.0000001L) break;
+    ++levels;
+  }
+  const long double target = powl((long double)N, 1.0L / levels);
+  int64_t best = 0;
+  for (int64_t d = 2; d <= cap && d <= N; ++d) {
+    if (N % d || !is_smooth7(d)) continue;
+    if ((long double)d <= target * 1.0000001L) best = d;
+  }
+  if (best == 0) {
+    for (int64_t d = std::min(cap, N); d >= 2; --d)
+      if (N % d == 0 && is_smooth7(d)) {
+        best = d;
+        break;
+      }
+  }
+  return best;
+}
+
+void PassProgram::emit(int64_t N, std::vector<Digit> lines, int64_t iks, int64_t oks,
+                       int sign, const ElemHook& ld, const ElemHook& st) {
+  const int64_t cap = lds_cap(prec_);
+  const int64_t L = prod_cnt(lines);
+  if (N <= cap && is_smooth7(N)) {
+    emit_single(N, lines, iks, oks, sign, ld, st, nullptr, -1, 0, L * N);
+    return;
+  }
+  if (!is_smooth7(N)) fail(ERR_INTERNAL, "emit: non-smooth length reached four-step");
+  // Four-step N = N1 * N2.
+  const int64_t N1 = choose_n1(N, cap);
+  const int64_t N2 = N / N1;
+  if (N1 < 2 || N2 < 2) fail(ERR_NOT_IMPL, "no four-step factorisation for this length");
+
+  // Pass A: N1-point DFTs along n1 (element stride N2*iks), one line per
+  // (lines..., n2); output in canonical [lines][k1][n2] with W_N^{n2 k1}.
+  std::vector<Digit> la = lines;
+  {
+    auto cs = canon_strides(lines, N);
+    for (size_t j = 0; j < la.size(); ++j) la[j].os = cs[j];
+    la.push_back(Digit{N2, iks, 1});
+  }
+  const int n2_level = (int)la.size() - 1;
+  // Parent index m = n * kmul + (digit terms); in pass A, n = n1 * N2 + n2.
+  ElemHook lda;
+  if (ld.active()) {
+    if (ld.level2 >= 0) fail(ERR_NOT_IMPL, "load hook deeper than two split levels");
+    lda = ld;
+    lda.kmul = N2 * ld.kmul;
+    if (ld.level >= 0) {
+      lda.level2 = n2_level;
+      lda.dmul2 = ld.kmul;
+    } else {
+      lda.level = n2_level;
+      lda.dmul = ld.kmul;
+    }
+  }
+  emit_single(N1, la, N2 * iks, N2, sign, lda, ElemHook{}, nullptr, n2_level, N, L * N);
+
+  // Job B: N2-point DFTs along n2 (unit stride in the intermediate), one line
+  // per (lines..., k1); the store writes X[k1 + N1 k2] in the caller layout.
+  std::vector<Digit> lb = lines;
+  {
+    auto cs = canon_strides(lines, N);
+    for (size_t j = 0; j < lb.size(); ++j) lb[j].is = cs[j];
+    lb.push_back(Digit{N1, N2, oks});
+  }
+  // Parent index m = k * kmul + (digit terms) with k = k1 + N1 * k2.
+  ElemHook stb;
+  if (st.active()) {
+    if (st.level2 >= 0) fail(ERR_NOT_IMPL, "store hook deeper than two split levels");
+    stb = st;
+    stb.kmul = N1 * st.kmul;
+    if (st.level >= 0) {
+      stb.level2 = (int)lb.size() - 1;
+      stb.dmul2 = st.kmul;
+    } else {
+      stb.level = (int)lb.size() - 1;
+      stb.dmul = st.kmul;
+    }
+  }
+  emit(N2, lb, 1, N1 * oks, sign, ElemHook{}, stb);
+}
+
+void PassProgram::add_transform(int64_t N, const std::vector<Digit>& lines, int64_t iks,
+                                int64_t oks) {
+  if (N < 1) fail(ERR_INVALID_DIM, "transform length < 1");
+  if (N == 1) return;  // identity
+  const int64_t cap = lds_cap(prec_);
+  const int64_t L = prod_cnt(lines);
+  if (is_smooth7(N)) {
+    emit(N, lines, iks, oks, 0, ElemHook{}, ElemHook{});
+    return;
+  }
+  const int64_t M = next_pow2(2 * N - 1);
+  // Direction-dependent chirp tables: [0] forward, [1] backward.
+  std::vector<cld> chirp[2], H[2];
+  bluestein_tables(N, M, -1, chirp[0], H[0]);
+  bluestein_tables(N, M, +1, chirp[1], H[1]);
+  const void* tc[2] = {upload(chirp[0]), upload(chirp[1])};
+  const void* th[2] = {upload(H[0]), upload(H[1])};
+  const size_t first = passes_.size();
+  if (M <= cap) {
+    ElemHook ld, st;
+    ld.table = tc[0];
+    ld.limit = N;
+    st.table = tc[0];
+    st.limit = N;
+    emit_single(M, lines, iks, oks, -1, ld, st, th[0], -1, 0, L * N);
+    PassSpec& ps = passes_.back();
+    ps.sign = 0;
+    ps.p.n = (int32_t)M;
+    ps.dir_tables = true;
+    ps.pre_t[0] = tc[0];
+    ps.pre_t[1] = tc[1];
+    ps.post_t[0] = tc[0];
+    ps.post_t[1] = tc[1];
+    ps.blu_t[0] = th[0];
+    ps.blu_t[1] = th[1];
+    return;
+  }
+  // Long Bluestein: forward M-FFT (chirp + zero padding on its first load,
+  // filter product on its last store), then backward M-FFT (output chirp and
+  // truncation on its last store).  The intermediate is canonical [lines][M].
+  std::vector<Digit> lf = lines;
+  auto cs = canon_strides(lines, M);
+  for (size_t j = 0; j < lf.size(); ++j) lf[j].os = cs[j];
+  ElemHook ld;
+  ld.table = tc[0];
+  ld.limit = N;
+  ElemHook sth;
+  sth.table = th[0];
+  emit(M, lf, iks, 1, -1, ld, sth);
+  std::vector<Digit> lbk = lines;
+  for (size_t j = 0; j < lbk.size(); ++j) lbk[j].is = cs[j];
+  ElemHook st;
+  st.table = tc[0];
+  st.limit = N;
+  emit(M, lbk, 1, oks, +1, ElemHook{}, st);
+  // Mark direction-dependent tables and intermediate sizes.
+  for (size_t i = first; i < passes_.size(); ++i) {
+    PassSpec& ps = passes_[i];
+    ps.out_elems = (i + 1 == passes_.size()) ? L * N : L * M;
+    if (ps.p.pre == tc[0] || ps.p.post == tc[0] || ps.p.post == th[0]) {
+      ps.dir_tables = true;
+      ps.pre_t[0] = ps.p.pre;
+      ps.pre_t[1] = ps.p.pre == tc[0] ? tc[1] : ps.p.pre;
+      ps.post_t[0] = ps.p.post;
+      ps.post_t[1] = ps.p.post == tc[0] ? tc[1] : (ps.p.post == th[0] ? th[1] : ps.p.post);
+      ps.blu_t[0] = ps.blu_t[1] = nullptr;
+    }
+  }
+}
+
+PassProgram::SlotMap PassProgram::assign(Mode mode, int64_t data_elems) const {
+  SlotMap m;
+  const size_t np = passes_.size();
+  m.in.assign(np, -1);
+  m.out.assign(np, -1);
+  if (np == 0) {
+    m.result = SLOT_DATA;
+    return m;
+  }
+  auto use_scratch = [&](int slot, int64_t elems) {
+    int i = slot - SLOT_S1;
+    m.scratch[i] = std::max(m.scratch[i], elems);
+  };
+  if (mode == PINGPONG) {
+    int cur = SLOT_DATA;
+    for (size_t i = 0; i < np; ++i) {
+      m.in[i] = cur;
+      int out;
+      if (passes_[i].out_elems <= data_elems) {
+        out = cur == SLOT_DATA ? SLOT_WORK : SLOT_DATA;
+      } else {
+        out = cur == SLOT_S1 ? SLOT_S2 : SLOT_S1;
+        use_scratch(out, passes_[i].out_elems);
+      }
+      m.out[i] = out;
+      cur = out;
+    }
+    m.result = cur;
+    return m;
+  }
+  // PRESERVE / INPLACE: last pass writes WORK; walk backwards alternating.
+  m.out[np - 1] = SLOT_WORK;
+  for (int i = (int)np - 2; i >= 0; --i) {
+    const int next_out = m.out[i + 1];
+    int out;
+    if (mode == PRESERVE && passes_[i].out_elems <= data_elems && next_out != SLOT_WORK)
+      out = SLOT_WORK;
+    else
+      out = next_out == SLOT_S1 ? SLOT_S2 : SLOT_S1;
+    if (out != SLOT_WORK) use_scratch(out, passes_[i].out_elems);
+    m.out[i] = out;
+  }
+  m.in[0] = SLOT_DATA;
+  for (size_t i = 1; i < np; ++i) m.in[i] = m.out[i - 1];
+  m.result = SLOT_WORK;
+  return m;
+}
+
+void PassProgram::run_one(const SlotMap& m, size_t i, void* const slots[SLOT_COUNT], int dir,
+                          hipStream_t stream) const {
+  const PassSpec& ps = passes_[i];
+  sa_pass_t p = ps.p;
+  p.in = slots[m.in[i]];
+  p.out = slots[m.out[i]];
+  if (!p.in || !p.out) fail(ERR_NO_BUFFER, "pass buffer not set");
+  if (ps.dir_tables) {
+    const int d = dir < 0 ? 0 : 1;
+    p.pre = ps.pre_t[d];
+    p.post = ps.post_t[d];
+    p.blu_h = ps.blu_t[d];
+  }
+  const int sign = ps.sign == 0 ? dir : ps.sign;
+  int rc = sa_launch_pass(&p, prec_, sign, stream);
+  hip_check((hipError_t)rc, "fft pass launch");
+}
+
+void PassProgram::run(const SlotMap& m, void* const slots[SLOT_COUNT], int dir,
+                      hipStream_t stream) const {
+  for (size_t i = 0; i < passes_.size(); ++i) run_one(m, i, slots, dir, stream);
+}
+
+int64_t PassProgram::total_bytes() const {
+  int64_t b = 0;
+  for (auto& p : passes_) b += p.bytes;
+  return b;
+}
+
+std::string PassProgram::describe() const {
+  std::ostringstream os;
+  for (size_t i = 0; i < passes_.size(); ++i) {
+    const sa_pass_t& p = passes_[i].p;
+    os << "pass " << i << ": n=" << p.n << " batch=" << p.batch << " radices=";
+    for (int s = 0; s < p.nstages; ++s) os << (s ? "x" : "") << p.radix[s];
+    os << " iks=" << p.iks << " oks=" << p.oks << " C=" << p.C << " T=" << p.T
+       << " ld=" << (p.ld_kfast ? "k" : "line") << " st=" << (p.st_kfast ? "k" : "line")
+       << (p.blu_h || passes_[i].blu_t[0] ? " bluestein" : "")
+       << (p.ptw_level >= 0 ? " twiddle" : "");
+    if (p.ld_level >= 0 || p.ld_limit < p.n || p.pre || passes_[i].pre_t[0])
+      os << " ld_hook(kmul=" << p.ld_kmul << ",lvl=" << p.ld_level << "/" << p.ld_level2
+         << ",lim=" << p.ld_limit << ")";
+    if (p.st_level >= 0 || p.st_limit < p.n || p.post || passes_[i].post_t[0])
+      os << " st_hook(kmul=" << p.st_kmul << ",lvl=" << p.st_level << "/" << p.st_level2
+         << ",lim=" << p.st_limit << ")";
+    os << "\n";
+  }
+  return os.str();
+}
+
+}  // namespace sa

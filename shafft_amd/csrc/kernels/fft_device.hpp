@@ -1,0 +1,231 @@
+// Device-side building blocks for the gfx950 FFT kernels: complex arithmetic,
+// fixed-radix DFT butterflies (2,3,4,5,7,8,16), and 32-bit division by a
+// run-time invariant.
+//
+// Conventions
+//   * SIGN = -1 is the forward transform  X[k] = sum_n x[n] exp(-2 pi i n k / N)
+//     (the sign SHAFFT uses: tests/unit/test_fft_partial.cpp:109-119),
+//     SIGN = +1 is the unnormalised backward transform.
+//   * Twiddle tables always hold forward roots exp(-2 pi i m / L); the
+//     backward kernels multiply by their conjugate.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sa {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+  R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> cadd(cpx<R> a, cpx<R> b) {
+  return {a.x + b.x, a.y + b.y};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> csub(cpx<R> a, cpx<R> b) {
+  return {a.x - b.x, a.y - b.y};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) {
+  return {a.x * s, a.y * s};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cconj(cpx<R> a) {
+  return {a.x, -a.y};
+}
+// a * w for the forward direction, a * conj(w) for the backward one.
+template <int SIGN, typename R>
+__device__ __forceinline__ cpx<R> cmul_dir(cpx<R> a, cpx<R> w) {
+  if constexpr (SIGN < 0) {
+    return {a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x};
+  } else {
+    return {a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y};
+  }
+}
+// a * exp(SIGN * i * pi / 2)
+template <int SIGN, typename R>
+__device__ __forceinline__ cpx<R> rot90(cpx<R> a) {
+  if constexpr (SIGN < 0) {
+    return {a.y, -a.x};
+  } else {
+    return {-a.y, a.x};
+  }
+}
+// a * (c + SIGN * i * s) with compile-time-known constants c, s.
+template <int SIGN, typename R>
+__device__ __forceinline__ cpx<R> crot(cpx<R> a, R c, R s) {
+  const R ss = SIGN < 0 ? -s : s;
+  return {a.x * c - a.y * ss, a.x * ss + a.y * c};
+}
+
+// ---------------------------------------------------------------------------
+// Butterflies: in-register DFT of length RAD, X[q] = sum_r v[r] w_RAD^{SIGN r q}.
+// ---------------------------------------------------------------------------
+
+template <int SIGN, typename R>
+__device__ __forceinline__ void dft2(cpx<R>* v) {
+  cpx<R> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int SIGN, typename R>
+__device__ __forceinline__ void dft4(cpx<R>& a0, cpx<R>& a1, cpx<R>& a2, cpx<R>& a3) {
+  cpx<R> t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  cpx<R> t2 = cadd(a1, a3), t3 = rot90<SIGN>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <int SIGN, typename R>
+__device__ __forceinline__ void dft4(cpx<R>* v) {
+  dft4<SIGN>(v[0], v[1], v[2], v[3]);
+}
+
+template <int SIGN, typename R>
+__device__ __forceinline__ void dft8(cpx<R>* v) {
+  constexpr R c = R(0.7071067811865475244008444);
+  // even / odd radix-4 halves
+  cpx<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cpx<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<SIGN>(e0, e1, e2, e3);
+  dft4<SIGN>(o0, o1, o2, o3);
+  o1 = crot<SIGN>(o1, c, c);
+  o2 = rot90<SIGN>(o2);
+  o3 = crot<SIGN>(o3, -c, c);
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1);
+  v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2);
+  v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3);
+  v[7] = csub(e3, o3);
+}
+
+template <int SIGN, typename R>
+__device__ __forceinline__ void dft16(cpx<R>* v) {
+  constexpr R c1 = R(0.9238795325112867561281832);  // cos(pi/8)
+  constexpr R s1 = R(0.3826834323650897717284600);  // sin(pi/8)
+  constexpr R c2 = R(0.7071067811865475244008444);  // cos(pi/4)
+  // Column DFT4s over r' for each r0 (r = 4 r' + r0).
+#pragma unroll
+  for (int r0 = 0; r0 < 4; ++r0) dft4<SIGN>(v[r0], v[r0 + 4], v[r0 + 8], v[r0 + 12]);
+  // v[r0 + 4 q0] now holds F_{r0}[q0]; twiddle by w16^{r0 q0}.
+  v[1 + 4 * 1] = crot<SIGN>(v[1 + 4 * 1], c1, s1);    // m=1
+  v[1 + 4 * 2] = crot<SIGN>(v[1 + 4 * 2], c2, c2);    // m=2
+  v[1 + 4 * 3] = crot<SIGN>(v[1 + 4 * 3], s1, c1);    // m=3
+  v[2 + 4 * 1] = crot<SIGN>(v[2 + 4 * 1], c2, c2);    // m=2
+  v[2 + 4 * 2] = rot90<SIGN>(v[2 + 4 * 2]);           // m=4
+  v[2 + 4 * 3] = crot<SIGN>(v[2 + 4 * 3], -c2, c2);   // m=6
+  v[3 + 4 * 1] = crot<SIGN>(v[3 + 4 * 1], s1, c1);    // m=3
+  v[3 + 4 * 2] = crot<SIGN>(v[3 + 4 * 2], -c2, c2);   // m=6
+  v[3 + 4 * 3] = crot<SIGN>(v[3 + 4 * 3], -c1, -s1);  // m=9
+  // Row DFT4s over r0 for each q0: X[q0 + 4 q1].
+  cpx<R> out[16];
+#pragma unroll
+  for (int q0 = 0; q0 < 4; ++q0) {
+    cpx<R> a0 = v[0 + 4 * q0], a1 = v[1 + 4 * q0], a2 = v[2 + 4 * q0], a3 = v[3 + 4 * q0];
+    dft4<SIGN>(a0, a1, a2, a3);
+    out[q0 + 0] = a0;
+    out[q0 + 4] = a1;
+    out[q0 + 8] = a2;
+    out[q0 + 12] = a3;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = out[i];
+}
+
+// Odd prime radix by the symmetric-pair formulation:
+//   X[q]   = A_q + SIGN i B_q,  X[R-q] = A_q - SIGN i B_q
+//   A_q = v0 + sum_r (v_r + v_{R-r}) cos(2 pi r q / R)
+//   B_q =      sum_r (v_r - v_{R-r}) sin(2 pi r q / R)
+template <int RAD>
+struct OddConst;
+template <>
+struct OddConst<3> {
+  static constexpr double c[2] = {1.0, -0.5};
+  static constexpr double s[2] = {0.0, 0.8660254037844386467637232};
+};
+template <>
+struct OddConst<5> {
+  static constexpr double c[3] = {1.0, 0.3090169943749474241022934, -0.8090169943749474241022934};
+  static constexpr double s[3] = {0.0, 0.9510565162951535721164393, 0.5877852522924731291687060};
+};
+template <>
+struct OddConst<7> {
+  static constexpr double c[4] = {1.0, 0.6234898018587335305250049, -0.2225209339563144042889026,
+                                  -0.9009688679024191262361023};
+  static constexpr double s[4] = {0.0, 0.7818314824680298087084445, 0.9749279121818236070181317,
+                                  0.4338837391175581204757683};
+};
+
+template <int RAD, int SIGN, typename R>
+__device__ __forceinline__ void dft_odd(cpx<R>* v) {
+  constexpr int H = (RAD - 1) / 2;
+  cpx<R> S[H + 1], D[H + 1];
+  cpx<R> x0 = v[0];
+  cpx<R> sum = x0;
+#pragma unroll
+  for (int r = 1; r <= H; ++r) {
+    S[r] = cadd(v[r], v[RAD - r]);
+    D[r] = csub(v[r], v[RAD - r]);
+    sum = cadd(sum, S[r]);
+  }
+  v[0] = sum;
+#pragma unroll
+  for (int q = 1; q <= H; ++q) {
+    cpx<R> A = x0, B = {R(0), R(0)};
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      const int m = (r * q) % RAD;
+      // cos(2 pi m/R) = c[min(m, R-m)], sin(2 pi m/R) = +-s[min(m, R-m)]
+      const int mm = m <= H ? m : RAD - m;
+      const R cm = R(OddConst<RAD>::c[mm]);
+      const R sm = m <= H ? R(OddConst<RAD>::s[mm]) : -R(OddConst<RAD>::s[mm]);
+      A.x += S[r].x * cm;
+      A.y += S[r].y * cm;
+      B.x += D[r].x * sm;
+      B.y += D[r].y * sm;
+    }
+    // SIGN i B
+    cpx<R> iB = SIGN < 0 ? cpx<R>{B.y, -B.x} : cpx<R>{-B.y, B.x};
+    v[q] = cadd(A, iB);
+    v[RAD - q] = csub(A, iB);
+  }
+}
+
+template <int RAD, int SIGN, typename R>
+__device__ __forceinline__ void dft(cpx<R>* v) {
+  if constexpr (RAD == 2) dft2<SIGN>(v);
+  else if constexpr (RAD == 4) dft4<SIGN>(v);
+  else if constexpr (RAD == 8) dft8<SIGN>(v);
+  else if constexpr (RAD == 16) dft16<SIGN>(v);
+  else dft_odd<RAD, SIGN>(v);
+}
+
+// ---------------------------------------------------------------------------
+// Division of a 31-bit unsigned value by a run-time invariant divisor using a
+// multiply-high and a shift (Granlund & Montgomery, PLDI 1994).  Host side
+// fills {d, m, s}; d == 1 is special-cased.
+// ---------------------------------------------------------------------------
+struct FastDiv {
+  uint32_t d;
+  uint32_t m;
+  uint32_t s;
+};
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
+  return f.d == 1 ? n : (__umulhi(n, f.m) >> f.s);
+}
+
+}  // namespace sa

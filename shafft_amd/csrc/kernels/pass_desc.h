@@ -1,0 +1,97 @@
+/* Descriptor of one LDS-resident FFT pass and the launch ABI between the
+ * host plan code (C++) and the HIP translation units.  Plain C so that both
+ * sides, and any future caller, see the same layout.
+ *
+ * A pass transforms `batch` lines of length `n`.  Line b is split into three
+ * mixed-radix digits (b0, b1, b2) with b2 = b % n2 fastest and
+ * b1 = (b / n2) % n1; element k of line b is read at
+ *     in [b0*is0 + b1*is1 + b2*is2 + k*iks]
+ * and written at
+ *     out[b0*os0 + b1*os1 + b2*os2 + k*oks]      (strides in complex elements).
+ * Independent in/out strides are what fuse the slab transposes of a
+ * multi-pass transform into the loads and stores of the adjacent pass.
+ */
+#ifndef SHAFFT_AMD_PASS_DESC_H
+#define SHAFFT_AMD_PASS_DESC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SA_MAX_STAGES 16
+
+typedef struct {
+  uint32_t d; /* divisor */
+  uint32_t m; /* magic multiplier */
+  uint32_t s; /* post-shift */
+} sa_fastdiv_t;
+
+typedef struct {
+  const void* in;
+  void* out;
+  int64_t batch;
+  int64_t n1, n2;
+  int64_t is0, is1, is2, iks;
+  int64_t os0, os1, os2, oks;
+
+  int32_t n;       /* transform length of this pass                    */
+  int32_t nstages; /* Stockham stages (0 => plain strided copy)        */
+  int32_t radix[SA_MAX_STAGES];
+  int32_t ns[SA_MAX_STAGES];    /* product of the radices before stage s   */
+  int32_t twoff[SA_MAX_STAGES]; /* offset (elements) into `tw` for stage s */
+  sa_fastdiv_t nsdiv[SA_MAX_STAGES];
+  const void* tw; /* per-stage forward twiddles, same precision as data   */
+
+  int32_t C;         /* lines per workgroup                              */
+  int32_t T;         /* threads per line                                 */
+  int32_t ld_kfast;  /* 1: lanes walk k on load, 0: lanes walk lines     */
+  int32_t st_kfast;  /* same for the store                               */
+  int32_t ldsstride; /* complex elements per line slot in LDS            */
+  sa_fastdiv_t Tdiv, Cdiv, n1div, n2div;
+
+  /* Optional post-twiddle on the store: out *= W_N^{+-(digit * k) mod N}
+   * with digit = (b0, b1, b2)[ptw_level]; W from a two-level table
+   * W^e = hi[e >> ptw_shift] * lo[e & ((1 << ptw_shift) - 1)].           */
+  int32_t ptw_level; /* -1: none */
+  int32_t ptw_shift;
+  int64_t ptw_N;
+  const void* ptw_lo;
+  const void* ptw_hi;
+
+  /* Element-index hooks.  For element k of line b with digits (b0,b1,b2):
+   *   m_ld = k * ld_kmul + digit[ld_level] * ld_dmul + digit[ld_level2] * ld_dmul2
+   * (a term is absent when its level is < 0)
+   * the element is read only if m_ld < ld_limit (else taken as zero) and,
+   * when `pre` is set, multiplied by pre[m_ld].  Likewise on the store with
+   * st_*: written only if m_st < st_limit, multiplied by post[m_st].
+   * Plain passes use kmul = 1, level = -1, limit = n.                     */
+  int64_t ld_kmul, ld_dmul, ld_dmul2, ld_limit;
+  int64_t st_kmul, st_dmul, st_dmul2, st_limit;
+  int32_t ld_level, ld_level2, st_level, st_level2;
+  const void* pre;
+  const void* post;
+  /* Bluestein mode: the pass runs forward stages, multiplies by blu_h[k]
+   * (FFT of the conjugate chirp, already divided by n), then runs backward
+   * stages, all inside LDS.  The direction is carried by pre/post.       */
+  const void* blu_h;
+
+  double scale; /* applied on the store when != 1                        */
+} sa_pass_t;
+
+/* Launch one pass.  prec: 0 = complex float, 1 = complex double.
+ * sign: -1 forward, +1 backward.  Returns a hipError_t value. */
+int sa_launch_pass(const sa_pass_t* p, int prec, int sign, void* stream);
+
+/* x[i] *= f for i < count complex elements. */
+int sa_launch_scale(void* data, double f, uint64_t count, int prec, void* stream);
+
+/* Largest dynamic LDS a pass may use (bytes). */
+int sa_max_lds_bytes(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,243 @@
+// CPU emulation of the pass launch ABI (kernels/pass_desc.h) and of the few
+// HIP runtime calls the host plan code makes.  Test infrastructure only: it
+// lets the planner's pass programs (digits, strides, four-step twiddles,
+// fused transposes, Bluestein hooks, buffer slots) be checked against numpy
+// on a machine without a GPU.  The Stockham stages are replayed with the
+// plan's own twiddle tables and radices; butterflies are direct DFTs.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <array>
+#include <cstdio>
+#include <map>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+
+#include "../../shafft_amd/csrc/kernels/pass_desc.h"
+
+namespace {
+
+// Allocation registry so every emulated access can be bounds-checked.
+std::map<uintptr_t, size_t>& allocs() {
+  static std::map<uintptr_t, size_t> m;
+  return m;
+}
+
+[[noreturn]] void oob(const char* what, const void* base, int64_t idx) {
+  std::fprintf(stderr, "EMU OUT OF BOUNDS: %s base=%p index=%lld\n", what, base, (long long)idx);
+  std::abort();
+}
+
+// Checks that element idx (of size esz) of the array at base lies inside one
+// registered allocation.
+void chk(const char* what, const void* base, int64_t idx, size_t esz) {
+  const uintptr_t a = (uintptr_t)base + (uintptr_t)(idx * (int64_t)esz);
+  if (idx < 0) oob(what, base, idx);
+  auto& m = allocs();
+  auto it = m.upper_bound(a);
+  if (it == m.begin()) oob(what, base, idx);
+  --it;
+  if (a + esz > it->first + it->second || (uintptr_t)base < it->first) oob(what, base, idx);
+}
+
+template <typename R>
+using C = std::complex<R>;
+
+template <typename R>
+C<R> conj_if(C<R> w, int sign) {
+  return sign < 0 ? w : std::conj(w);
+}
+
+template <typename R>
+void stages(const sa_pass_t& p, std::vector<C<R>>& d, int sign) {
+  const int n = p.n;
+  const C<R>* tw = reinterpret_cast<const C<R>*>(p.tw);
+  std::vector<C<R>> out(n);
+  for (int s = 0; s < p.nstages; ++s) {
+    const int Rr = p.radix[s], Ns = p.ns[s], nb = n / Rr;
+    const C<R>* tws = tw + p.twoff[s];
+    for (int jb = 0; jb < nb; ++jb) {
+      const int q = jb / Ns, rem = jb % Ns;
+      std::vector<C<R>> v(Rr), V(Rr);
+      for (int r = 0; r < Rr; ++r) {
+        v[r] = d.at(jb + r * nb);
+        if (Ns > 1 && r > 0) {
+          chk("tw", tws, rem * (Rr - 1) + r - 1, sizeof(C<R>));
+          v[r] *= conj_if(tws[rem * (Rr - 1) + r - 1], sign);
+        }
+      }
+      for (int k = 0; k < Rr; ++k) {
+        C<long double> acc = 0;
+        for (int r = 0; r < Rr; ++r) {
+          const long double a = sign * 2.0L * 3.14159265358979323846264338327950288L *
+                                (long double)((r * k) % Rr) / Rr;
+          acc += C<long double>(v[r].real(), v[r].imag()) * C<long double>(cosl(a), sinl(a));
+        }
+        V[k] = C<R>((R)acc.real(), (R)acc.imag());
+      }
+      const int idx = q * Ns * Rr + rem;
+      for (int r = 0; r < Rr; ++r) out.at(idx + r * Ns) = V[r];
+    }
+    d.swap(out);
+  }
+}
+
+template <typename R>
+void check_shape(const sa_pass_t& p) {
+  auto die = [](const char* m) {
+    std::fprintf(stderr, "EMU BAD LAUNCH: %s\n", m);
+    std::abort();
+  };
+  const int emax = 16;
+  if (p.T < 1 || p.C < 1 || p.C * p.T > 512) die("threads per workgroup");
+  if ((int64_t)p.n > (int64_t)emax * p.T) die("n > EMAX * T");
+  for (int s = 0; s < p.nstages; ++s) {
+    const int rad = p.radix[s];
+    const int maxb = (emax + rad - 1) / rad;
+    if ((p.n / rad + p.T - 1) / p.T > maxb) die("butterflies per thread exceed register budget");
+  }
+  if (p.ldsstride < p.n) die("lds stride");
+  if ((int64_t)p.C * p.ldsstride * (int64_t)sizeof(C<R>) > 160 * 1024) die("lds bytes");
+  if (p.batch >= (int64_t(1) << 31)) die("batch");
+  if (p.ptw_level > 2 || p.ld_level > 2 || p.st_level > 2 || p.ld_level2 > 2 || p.st_level2 > 2)
+    die("level");
+}
+
+template <typename R>
+void run_pass(const sa_pass_t& p, int sign) {
+  check_shape<R>(p);
+  const int n = p.n;
+  const C<R>* in = reinterpret_cast<const C<R>*>(p.in);
+  C<R>* outp = reinterpret_cast<C<R>*>(p.out);
+  const C<R>* pre = reinterpret_cast<const C<R>*>(p.pre);
+  const C<R>* post = reinterpret_cast<const C<R>*>(p.post);
+  // Like the GPU, all lines are loaded before any is stored only within a
+  // workgroup; emulate workgroup granularity so aliasing bugs show up.
+  const int64_t Cw = p.C;
+  for (int64_t b0 = 0; b0 < p.batch; b0 += Cw) {
+    const int64_t bend = std::min<int64_t>(p.batch, b0 + Cw);
+    std::vector<std::vector<C<R>>> lines;
+    std::vector<int64_t> ooff;
+    std::vector<std::array<int64_t, 3>> digs;
+    for (int64_t b = b0; b < bend; ++b) {
+      const int64_t b2 = b % p.n2, q = b / p.n2, b1 = q % p.n1, bb0 = q / p.n1;
+      const int64_t ioff = bb0 * p.is0 + b1 * p.is1 + b2 * p.is2;
+      ooff.push_back(bb0 * p.os0 + b1 * p.os1 + b2 * p.os2);
+      std::array<int64_t, 3> dg = {bb0, b1, b2};
+      digs.push_back(dg);
+      std::vector<C<R>> d(n);
+      for (int k = 0; k < n; ++k) {
+        const int64_t m = (int64_t)k * p.ld_kmul + (p.ld_level >= 0 ? dg[p.ld_level] * p.ld_dmul : 0) +
+                          (p.ld_level2 >= 0 ? dg[p.ld_level2] * p.ld_dmul2 : 0);
+        C<R> v = 0;
+        if (m < p.ld_limit) {
+          chk("in", in, ioff + (int64_t)k * p.iks, sizeof(C<R>));
+          v = in[ioff + (int64_t)k * p.iks];
+          if (pre) {
+            chk("pre", pre, m, sizeof(C<R>));
+            v *= pre[m];
+          }
+        }
+        d[k] = v;
+      }
+      if (!p.blu_h) {
+        stages<R>(p, d, sign);
+      } else {
+        stages<R>(p, d, -1);
+        const C<R>* h = reinterpret_cast<const C<R>*>(p.blu_h);
+        for (int k = 0; k < n; ++k) {
+          chk("blu_h", h, k, sizeof(C<R>));
+          d[k] *= h[k];
+        }
+        stages<R>(p, d, +1);
+      }
+      lines.push_back(std::move(d));
+    }
+    for (int64_t b = b0; b < bend; ++b) {
+      const auto& d = lines[b - b0];
+      const auto& dg = digs[b - b0];
+      for (int k = 0; k < n; ++k) {
+        C<R> v = d[k];
+        if (p.ptw_level >= 0) {
+          const uint64_t e = ((uint64_t)dg[p.ptw_level] * (uint64_t)k) % (uint64_t)p.ptw_N;
+          const C<R>* lo = reinterpret_cast<const C<R>*>(p.ptw_lo);
+          const C<R>* hi = reinterpret_cast<const C<R>*>(p.ptw_hi);
+          chk("ptw_hi", hi, (int64_t)(e >> p.ptw_shift), sizeof(C<R>));
+          chk("ptw_lo", lo, (int64_t)(e & ((1ull << p.ptw_shift) - 1)), sizeof(C<R>));
+          const C<R> w = hi[e >> p.ptw_shift] * lo[e & ((1ull << p.ptw_shift) - 1)];
+          v *= conj_if(w, sign);
+        } else {
+          const int64_t m = (int64_t)k * p.st_kmul + (p.st_level >= 0 ? dg[p.st_level] * p.st_dmul : 0) +
+                            (p.st_level2 >= 0 ? dg[p.st_level2] * p.st_dmul2 : 0);
+          if (m >= p.st_limit) continue;
+          if (post) {
+            chk("post", post, m, sizeof(C<R>));
+            v *= post[m];
+          }
+        }
+        if (p.scale != 1.0) v *= (R)p.scale;
+        chk("out", outp, ooff[b - b0] + (int64_t)k * p.oks, sizeof(C<R>));
+        outp[ooff[b - b0] + (int64_t)k * p.oks] = v;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int sa_max_lds_bytes(void) { return 160 * 1024; }
+
+extern "C" int sa_launch_pass(const sa_pass_t* p, int prec, int sign, void*) {
+  if (prec == 0)
+    run_pass<float>(*p, sign);
+  else
+    run_pass<double>(*p, sign);
+  return 0;
+}
+
+extern "C" int sa_launch_scale(void* data, double f, uint64_t count, int prec, void*) {
+  if (prec == 0) {
+    float* x = (float*)data;
+    const float ff = (float)f;
+    for (uint64_t i = 0; i < 2 * count; ++i) x[i] *= ff;
+  } else {
+    double* x = (double*)data;
+    for (uint64_t i = 0; i < 2 * count; ++i) x[i] *= f;
+  }
+  return 0;
+}
+
+// ---- the HIP runtime calls used by the host code, on host memory ----------
+extern "C" {
+hipError_t hipMalloc(void** p, size_t n) {
+  *p = std::malloc(n ? n : 1);
+  if (*p) allocs()[(uintptr_t)*p] = n;
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipFree(void* p) {
+  allocs().erase((uintptr_t)p);
+  std::free(p);
+  return hipSuccess;
+}
+hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
+  std::memmove(d, s, n);
+  return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+  std::memmove(d, s, n);
+  return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+const char* hipGetErrorString(hipError_t) { return "emulated"; }
+hipError_t hipGetDevice(int* d) {
+  *d = 0;
+  return hipSuccess;
+}
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
+  *v = (a == hipDeviceAttributeMaxSharedMemoryPerBlock) ? 160 * 1024 : 0;
+  return hipSuccess;
+}
+}
