@@ -1,0 +1,218 @@
+"""Pass programs of the planner, run through the CPU emulation build.
+
+tests/emu/emu_pass.cpp replays each pass descriptor on the host with the
+plan's own twiddle tables and radices, bounds-checks every access against
+its allocation and checks the launch-shape assumptions of the kernel.  These
+tests pin the planner's index arithmetic (digits, strides, fused
+transposes, four-step twiddles, Bluestein hooks, buffer slots) against the
+oracle without a GPU.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from shafft_amd import api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TOL = {0: 1e-5, 1: 1e-12}
+DT = {0: np.complex64, 1: np.complex128}
+
+
+def nd_roundtrip(L, dims, axes, prec, seed=1):
+    x = oracle.uniform_input(dims, seed, DT[prec])
+    p = api.NDBackendPlan(dims, axes, prec, L=L)
+    n = x.size
+    a, b = api.DeviceBuffer(n, prec, L), api.DeviceBuffer(n, prec, L)
+    out = {}
+    for d, sign in ((api.FORWARD, -1), (api.BACKWARD, 1)):
+        a.upload(x.ravel())
+        dd, ww = p.execute(a.ptr, b.ptr, d)
+        y = api.download_ptr(dd, n, prec, L).reshape(dims)
+        out[d] = (y, oracle.dftn(x.astype(np.complex128), axes, sign))
+    return p, out
+
+
+ND_CASES = [
+    ((64, 64, 32), 0, None), ((64, 64, 32), 1, None), ((2, 3, 4, 5), 1, (0, 1, 2)),
+    ((4, 3, 2, 3, 2), 1, (1, 2, 3)), ((2, 3, 4, 5), 1, (2, 3)), ((3, 2, 2, 2, 2), 1, (1, 2, 3, 4)),
+    ((63, 47, 31), 1, None), ((5, 11), 0, None), ((1, 1, 1), 1, None), ((128, 64), 0, None),
+    ((360, 3), 1, None), ((12, 8192), 0, (1,)), ((2, 1009), 0, None), ((7, 5, 3, 2, 2, 2), 1, None),
+]
+
+
+@pytest.mark.parametrize("dims,prec,axes", ND_CASES)
+def test_nd_plans(emu_lib, dims, prec, axes):
+    axes = tuple(range(len(dims))) if axes is None else axes
+    p, out = nd_roundtrip(emu_lib, dims, list(axes), prec)
+    for d, (y, ref) in out.items():
+        assert oracle.check_rel_error(y, ref, TOL[prec]), (dims, d, oracle.rel_l2(y, ref))
+
+
+def _run_in_subprocess(code, env_extra):
+    env = dict(os.environ)
+    env.update(env_extra)
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+SUB = r"""
+import sys, numpy as np
+sys.path.insert(0, '.')
+import oracle
+from shafft_amd import api
+L = api.bind('_build/emu/libshafft_emu.so')
+for n, prec in CASES:
+    p = api.OneDBackendPlan(n, prec, L=L)
+    x = oracle.uniform_input((n,), 44, np.complex64 if prec == 0 else np.complex128)
+    a = api.DeviceBuffer(n, prec, L); b = api.DeviceBuffer(n, prec, L); a.upload(x)
+    for d in (0, 1):
+        p.execute(a.ptr, b.ptr, d)
+        y = b.download()
+        assert np.array_equal(a.download(), x), 'input not preserved'
+        r = np.fft.fft(x.astype(complex)) if d == 0 else np.fft.ifft(x.astype(complex)) * n
+        e = np.linalg.norm(y - r) / np.linalg.norm(r)
+        print(n, prec, d, p.num_passes(), e)
+        assert e < (1e-5 if prec == 0 else 1e-12), (n, prec, d, e)
+"""
+
+
+def test_1d_plans_hardware_cap(emu_lib):
+    out = _run_in_subprocess(SUB.replace("CASES", "[(1,1),(7,0),(4096,1),(8192,0),(16384,1),(100000,1),(40009,1)]"), {})
+    assert "40009 1 1 4" in out  # long Bluestein: 4 passes
+
+
+def test_1d_plans_small_cap_multilevel(emu_lib):
+    # A tiny LDS cap forces three-level four-step and three-level Bluestein.
+    out = _run_in_subprocess(SUB.replace("CASES", "[(4096,1),(32768,1),(1000,0),(997,1),(30030,1),(6561,1)]"),
+                             {"SHAFFT_AMD_LDS_CAP": "64"})
+    assert "30030 1 0 6" in out
+
+
+def test_nd_small_cap_multilevel(emu_lib):
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, '.')
+import oracle
+from shafft_amd import api
+L = api.bind('_build/emu/libshafft_emu.so')
+for dims in [(64, 48, 30), (7, 1000), (3, 257, 2)]:
+    p = api.NDBackendPlan(dims, list(range(len(dims))), 1, L=L)
+    x = oracle.uniform_input(dims, 45)
+    a = api.DeviceBuffer(x.size, 1, L); b = api.DeviceBuffer(x.size, 1, L); a.upload(x.ravel())
+    d, w = p.execute(a.ptr, b.ptr, 0)
+    y = api.download_ptr(d, x.size, 1, L).reshape(dims)
+    e = oracle.rel_l2(y, np.fft.fftn(x))
+    print(dims, p.num_passes(), e)
+    assert e < 1e-12
+"""
+    _run_in_subprocess(code, {"SHAFFT_AMD_LDS_CAP": "32"})
+
+
+def test_inplace_buffers(emu_lib):
+    L = emu_lib
+    dims = (16, 12, 10)
+    x = oracle.uniform_input(dims, 2)
+    p = api.NDBackendPlan(dims, [0, 1, 2], 1, L=L)
+    a = api.DeviceBuffer(x.size, 1, L)
+    a.upload(x.ravel())
+    d, w = p.execute(a.ptr, a.ptr, 0)
+    assert d == a.ptr
+    assert oracle.rel_l2(a.download().reshape(dims), oracle.dftn(x)) < 1e-13
+    p1 = api.OneDBackendPlan(4096 * 3, 1, L=L)
+    y = oracle.uniform_input((4096 * 3,), 3)
+    a1 = api.DeviceBuffer(y.size, 1, L)
+    a1.upload(y)
+    p1.execute(a1.ptr, a1.ptr, 1)
+    assert oracle.rel_l2(a1.download(), np.fft.ifft(y) * y.size) < 1e-13
+
+
+def test_c_api_state_machine_and_errors(emu_lib):
+    """Status codes of the reference's state machine (src/cpp/shafft.cpp:55-75,
+    172-186, shafft_internal.cpp:198-210,229-333)."""
+    L = emu_lib
+    import ctypes as ct
+    h = ct.c_void_p()
+    assert L.shafftNDCreate(ct.byref(h)) == 0
+    assert L.shafftPlan(h) == 4            # not configured -> ERR_PLAN_NOT_INIT
+    assert L.shafftExecute(h, 0) == 4
+    dims = (ct.c_int * 3)(8, 8, 8)
+    bad = (ct.c_int * 3)(2, 1, 1)
+    assert L.shafftNDInit(h, 3, bad, dims, 0, 0, 0) == 7   # grid > 1 rank -> ERR_INVALID_DECOMP
+    last = (ct.c_int * 3)(1, 1, 2)
+    assert L.shafftNDInit(h, 3, last, dims, 0, 0, 0) == 7  # last axis distributed
+    ones = (ct.c_int * 3)(1, 1, 1)
+    assert L.shafftNDInit(h, 3, ones, dims, 5, 0, 0) == 8  # bad type
+    zero = (ct.c_int * 3)(8, 0, 8)
+    assert L.shafftNDInit(h, 3, ones, zero, 0, 0, 0) == 5  # bad dim
+    assert L.shafftNDInit(h, 3, ones, dims, 0, 7, 0) == 2  # communicator other than one rank
+    assert L.shafftNDInit(h, 3, ones, dims, 0, 0, 0) == 0
+    assert L.shafftNDInit(h, 3, ones, dims, 0, 0, 0) == 15  # twice -> ERR_INVALID_STATE
+    assert L.shafftSetStream(h, None) == 4                  # before plan()
+    assert L.shafftPlan(h) == 0
+    assert L.shafftPlan(h) == 15
+    assert L.shafftExecute(h, 0) == 3                       # no buffers
+    n = ct.c_size_t()
+    assert L.shafftGetAllocSize(h, ct.byref(n)) == 0 and n.value == 512
+    sub = (ct.c_size_t * 3)()
+    off = (ct.c_size_t * 3)()
+    assert L.shafftGetLayout(h, sub, off, 0) == 0 and list(sub) == [8, 8, 8] and list(off) == [0, 0, 0]
+    ca = (ct.c_int * 3)()
+    assert L.shafftGetAxes(h, ca, None, 0) == 0 and list(ca) == [0, 1, 2]
+    assert L.shafftLastErrorStatus() == 0
+    assert L.shafftDestroy(ct.byref(h)) == 0 and h.value is None
+    assert L.shafftDestroy(ct.byref(h)) == 1
+    assert L.shafftLastErrorStatus() == 1
+    assert L.shafftErrorSourceName(2) == b"HIP"
+    h1 = ct.c_void_p()
+    assert L.shafft1DCreate(ct.byref(h1)) == 0
+    assert L.shafft1DInit(h1, 100, 50, 0, 1, 0) == 12       # layout of another rank count
+    assert L.shafft1DInit(h1, 100, 100, 0, 1, 0) == 0
+    ln, ls, la = ct.c_size_t(), ct.c_size_t(), ct.c_size_t()
+    assert L.shafftConfiguration1D(100, ct.byref(ln), ct.byref(ls), ct.byref(la), 1, 0) == 0
+    assert (ln.value, ls.value, la.value) == (100, 0, 100)
+    assert L.shafftDestroy(ct.byref(h1)) == 0
+
+
+def test_c_api_roundtrip_and_normalize(emu_lib):
+    L = emu_lib
+    for prec in (0, 1):
+        dims = (8, 6, 4)
+        x = oracle.uniform_input(dims, 42, DT[prec])
+        plan = api.FFTND(L=L)
+        plan.init([1, 1, 1], dims, prec)
+        plan.plan()
+        n = plan.alloc_size()
+        a, b = api.DeviceBuffer(n, prec, L), api.DeviceBuffer(n, prec, L)
+        a.upload(x.ravel())
+        plan.set_buffers(a.ptr, b.ptr)
+        plan.execute(api.FORWARD)
+        d, w = plan.get_buffers()
+        assert (d, w) == (b.ptr, a.ptr)   # odd pass count: result behind data after the swap
+        plan.execute(api.BACKWARD)
+        plan.normalize()                  # exponent 2 -> 1/N
+        d, _ = plan.get_buffers()
+        y = api.download_ptr(d, n, prec, L).reshape(dims)
+        assert oracle.check_rel_error(y, x, TOL[prec])
+        f = oracle.norm_factor_nd(dims, 2, prec)
+        assert f == np.float32(1 / 192) if prec == 0 else f == 1 / 192
+    # FFT1D: data -> work then swap; normalize factor in working precision
+    x = oracle.uniform_input((100,), 44)
+    p = api.FFT1D(L=L)
+    p.init(100, 100, 0, 1)
+    p.plan()
+    a, b = api.DeviceBuffer(100, 1, L), api.DeviceBuffer(100, 1, L)
+    a.upload(x)
+    p.set_buffers(a.ptr, b.ptr)
+    p.execute(api.FORWARD)
+    d, w = p.get_buffers()
+    assert (d, w) == (b.ptr, a.ptr)
+    assert np.array_equal(api.download_ptr(w, 100, 1, L), x)  # input preserved
+    p.normalize()
+    y = api.download_ptr(d, 100, 1, L)
+    assert oracle.rel_l2(y, np.fft.fft(x) / 10) < 1e-14
