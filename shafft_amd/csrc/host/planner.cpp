@@ -92,6 +92,23 @@ std::vector<int> radix_plan(int64_t n) {
   return r;
 }
 
+// Radix sequence the fixed-length kernel is compiled with (fft_fixed.hip,
+// Radices<N>); its twiddle table must follow the same sequence.
+static std::vector<int> fixed_radix_plan(int64_t n) {
+  switch (n) {
+    case 32: return {8, 4};
+    case 64: return {8, 8};
+    case 128: return {16, 8};
+    case 256: return {16, 16};
+    case 512: return {8, 8, 8};
+    case 1024: return {16, 8, 8};
+    case 2048: return {16, 16, 8};
+    case 4096: return {16, 16, 16};
+    case 8192: return {16, 16, 8, 4};
+    default: return {};
+  }
+}
+
 static int64_t next_pow2(int64_t x) {
   int64_t m = 1;
   while (m < x) m <<= 1;
@@ -160,8 +177,20 @@ static std::vector<Digit> merge_digits(const std::vector<Digit>& in, std::vector
   return d;
 }
 
-// Fill launch shape: threads per line, lines per workgroup, lane mappings.
-static void configure_launch(sa_pass_t& p, int elem_bytes) {
+// Tuning knobs read at plan time (used by the schedule sweep and tests):
+//   SHAFFT_AMD_LF_LINES  lines per workgroup for passes with a line-fast side
+//   SHAFFT_AMD_NT        1: non-temporal data loads/stores
+//   SHAFFT_AMD_REGSTAGE  0: disable the register-resident first/last stage
+static int env_int(const char* name, int dflt) {
+  const char* e = std::getenv(name);
+  return e && *e ? std::atoi(e) : dflt;
+}
+
+static bool pow2_le16(int r) { return r == 2 || r == 4 || r == 8 || r == 16; }
+
+// Fill launch shape: threads per line, lines per workgroup, lane mappings,
+// LDS stride and the register-resident stage flags.
+static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
   const int n = p.n;
   const int emax = 16;
   int T = (n + emax - 1) / emax;
@@ -173,12 +202,17 @@ static void configure_launch(sa_pass_t& p, int elem_bytes) {
   p.st_kfast = st_lf ? 0 : 1;
   const bool lf = ld_lf || st_lf;
   int C = std::max(1, 256 / T);
-  if (lf) C = std::max(C, elem_bytes == 8 ? 16 : 8);
-  // line-fast lanes walk lines: keep C within the fast digit
-  if (lf) C = (int)std::min<int64_t>(C, p.n2);
+  if (lf) {
+    C = std::max(C, elem_bytes == 8 ? 16 : 8);
+    const int knob = env_int("SHAFFT_AMD_LF_LINES", 0);
+    if (knob > 0) C = knob;
+    // line-fast lanes walk lines: keep C within the fast digit
+    C = (int)std::min<int64_t>(C, p.n2);
+  }
   while (C > 1 && C * T > 512) C /= 2;
-  int stride = n;
-  if (lf && (n % 2 == 0)) stride = n + 1;
+  // padded row: index k -> k + k/16, odd stride so line-fast lanes spread
+  int stride = n + (n - 1) / 16;
+  if (stride % 2 == 0) stride += 1;
   const int budget = lds_budget_bytes();
   while (C > 1 && (int64_t)C * stride * elem_bytes > budget) C /= 2;
   if ((int64_t)C * stride * elem_bytes > budget) fail(ERR_INTERNAL, "pass does not fit LDS");
@@ -188,6 +222,68 @@ static void configure_launch(sa_pass_t& p, int elem_bytes) {
   p.ldsstride = stride;
   p.Tdiv = make_fastdiv((uint32_t)T);
   p.Cdiv = make_fastdiv((uint32_t)C);
+  const bool regs_ok = !bluestein && env_int("SHAFFT_AMD_REGSTAGE", 1) != 0 && n == emax * T &&
+                       p.nstages >= 2;
+  // Register-resident first and last stage (one kernel variant): needs
+  // radices dividing 16 at both ends and one lane mapping for load and store.
+  const bool regs = regs_ok && pow2_le16(p.radix[0]) && pow2_le16(p.radix[p.nstages - 1]) &&
+                    p.ld_kfast == p.st_kfast;
+  p.reg_first = regs ? 1 : 0;
+  p.reg_last = regs ? 1 : 0;
+  p.nt = env_int("SHAFFT_AMD_NT", 1) ? 1 : 0;
+}
+
+static int device_cus() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 0;
+    cached = 256;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      cached = v;
+  }
+  return cached;
+}
+
+// Switch a plain power-of-two pass to the fixed-length persistent kernel:
+// lines per workgroup C (power of two), LDS stride and persistent grid.
+//   SHAFFT_AMD_FIXED=0 disables it; SHAFFT_AMD_FX_C / SHAFFT_AMD_FX_CK
+//   override C for passes with / without a line-fast side.
+static bool configure_fixed(sa_pass_t& p, int prec, int64_t& grid) {
+  const int n = p.n;
+  if (!env_int("SHAFFT_AMD_FIXED", 1) || !sa_fixed_supported(n)) return false;
+  if (p.pre || p.post || p.blu_h || p.ld_level >= 0 || p.st_level >= 0 || p.ld_kmul != 1 ||
+      p.st_kmul != 1 || p.ld_limit < n || p.st_limit < n)
+    return false;
+  const int esz = prec ? 16 : 8;
+  const int T = n / sa_fixed_elems(prec);
+  if (T < 1) return false;
+  const bool lf = !p.ld_kfast || !p.st_kfast;
+  int C = lf ? 16 : std::max(1, 256 / T);
+  const int knob = env_int(lf ? "SHAFFT_AMD_FX_C" : "SHAFFT_AMD_FX_CK", 0);
+  if (knob > 0) C = knob;
+  int logC = 0;
+  while ((1 << (logC + 1)) <= C) ++logC;  // round down to a power of two
+  int stride = n + (n - 1) / 16;
+  if (stride % 2 == 0) stride += 1;
+  const int budget = lds_budget_bytes();
+  const int64_t maxt = sa_fixed_max_threads(prec);
+  while (logC > 0 && (((int64_t)T << logC) > maxt || ((int64_t)stride << logC) * esz > budget)) --logC;
+  if (((int64_t)stride << logC) * esz > budget || ((int64_t)T << logC) > maxt) return false;
+  const int nthr = T << logC;
+  const size_t lds = ((size_t)stride << logC) * esz;
+  const int occ = sa_fixed_occupancy(n, prec, -1, nthr, lds);
+  if (occ <= 0) return false;
+  const int64_t tiles = (p.batch + (1 << logC) - 1) >> logC;
+  grid = std::min<int64_t>(tiles, (int64_t)occ * device_cus());
+  p.logC = logC;
+  p.C = 1 << logC;
+  p.T = T;
+  p.ldsstride = stride;
+  p.Tdiv = make_fastdiv((uint32_t)T);
+  p.Cdiv = make_fastdiv((uint32_t)p.C);
+  p.reg_first = p.reg_last = 0;
+  return true;
 }
 
 void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, int64_t oks,
@@ -262,7 +358,11 @@ void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, 
   p.n1div = make_fastdiv((uint32_t)p.n1);
   p.n2div = make_fastdiv((uint32_t)p.n2);
 
-  std::vector<int> rad = N > 1 ? radix_plan(N) : std::vector<int>{};
+  // The fixed-length kernel (when eligible) dictates the radix sequence.
+  const bool fixed_candidate = env_int("SHAFFT_AMD_FIXED", 1) && sa_fixed_supported((int)N) &&
+                               blu_h == nullptr && !ld.active() && !st.active();
+  std::vector<int> rad = N > 1 ? (fixed_candidate ? fixed_radix_plan(N) : radix_plan(N))
+                               : std::vector<int>{};
   if (rad.size() > SA_MAX_STAGES) fail(ERR_INTERNAL, "too many stages");
   std::vector<int> offs;
   p.tw = stage_table(rad, offs);
@@ -301,7 +401,14 @@ void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, 
   p.post = st.table;
   p.blu_h = blu_h;
   p.scale = 1.0;
-  configure_launch(p, prec_ ? 16 : 8);
+  configure_launch(p, prec_ ? 16 : 8, blu_h != nullptr);
+  {
+    int64_t grid = 0;
+    if (configure_fixed(p, prec_, grid)) {
+      ps.fixed = true;
+      ps.grid = (int)grid;
+    }
+  }
 
   ps.sign = sign;
   ps.out_elems = out_elems;
@@ -551,7 +658,8 @@ void PassProgram::run_one(const SlotMap& m, size_t i, void* const slots[SLOT_COU
     p.blu_h = ps.blu_t[d];
   }
   const int sign = ps.sign == 0 ? dir : ps.sign;
-  int rc = sa_launch_pass(&p, prec_, sign, stream);
+  int rc = ps.fixed ? sa_launch_fixed(&p, prec_, sign, ps.grid, stream)
+                    : sa_launch_pass(&p, prec_, sign, stream);
   hip_check((hipError_t)rc, "fft pass launch");
 }
 
@@ -570,7 +678,8 @@ std::string PassProgram::describe() const {
   std::ostringstream os;
   for (size_t i = 0; i < passes_.size(); ++i) {
     const sa_pass_t& p = passes_[i].p;
-    os << "pass " << i << ": n=" << p.n << " batch=" << p.batch << " radices=";
+    os << "pass " << i << ": n=" << p.n << " batch=" << p.batch
+       << (passes_[i].fixed ? " fixed(grid=" + std::to_string(passes_[i].grid) + ")" : "") << " radices=";
     for (int s = 0; s < p.nstages; ++s) os << (s ? "x" : "") << p.radix[s];
     os << " iks=" << p.iks << " oks=" << p.oks << " C=" << p.C << " T=" << p.T
        << " ld=" << (p.ld_kfast ? "k" : "line") << " st=" << (p.st_kfast ? "k" : "line")

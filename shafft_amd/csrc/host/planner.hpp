@@ -56,6 +56,9 @@ struct PassSpec {
   int sign = 0;  // 0: direction of the execute call, else fixed -1/+1
   int64_t out_elems = 0;  // elements of the buffer it writes (for slot sizing)
   int64_t bytes = 0;  // algorithmic HBM bytes (one read + one write per element)
+  // Fixed-length persistent kernel and its grid (workgroups).
+  bool fixed = false;
+  int grid = 0;
   // Bluestein passes carry direction-dependent tables: [0] forward, [1] backward.
   bool dir_tables = false;
   const void* pre_t[2] = {nullptr, nullptr};

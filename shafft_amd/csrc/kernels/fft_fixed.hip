@@ -1,0 +1,423 @@
+// Fixed-length persistent FFT pass for gfx950: power-of-two N known at
+// compile time (32 .. 8192), the hot shapes of the slab path.
+//
+// Same pass semantics as fft_pass.hip (pass_desc.h addressing, four-step
+// twiddle, scale), specialised so that
+//   * every Stockham index is a compile-time shift/or, and LDS addresses
+//     are one base register plus immediate offsets;
+//   * each thread owns E elements (k = tl + T*m, T = N/E threads per line;
+//     E = 16 complex doubles or 32 complex floats); the first stage runs on
+//     the loaded registers and, when load and store use the same lane
+//     mapping, the last stage writes its butterfly outputs straight to HBM;
+//   * the grid is persistent: each workgroup walks tiles b, b+G, b+2G, ...
+//     and issues the HBM loads of its next tile before running the LDS
+//     stages of the current one, so memory traffic overlaps the exchanges.
+// One tile = C lines (C a power of two).  HBM-bound: 2 * N * sizeof(T)
+// algorithmic bytes per line.
+#include "fft_device.hpp"
+#include "pass_desc.h"
+
+namespace sa {
+namespace fx {
+
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+// Elements per thread (16) and the workgroup size bound: 512 threads for
+// complex double (256 VGPRs per lane), 1024 for complex float (128 VGPRs),
+// so a 16-line tile of a 512/1024-point line is one workgroup.
+template <typename R>
+struct Elems {
+  static constexpr int E = 16;
+  static constexpr int MAXT = sizeof(R) == 8 ? 512 : 1024;
+};
+
+// Radix sequence for N (each radix divides 16).
+template <int N>
+struct Radices;
+template <> struct Radices<32> { static constexpr int n = 2, r[4] = {8, 4, 0, 0}; };
+template <> struct Radices<64> { static constexpr int n = 2, r[4] = {8, 8, 0, 0}; };
+template <> struct Radices<128> { static constexpr int n = 2, r[4] = {16, 8, 0, 0}; };
+template <> struct Radices<256> { static constexpr int n = 2, r[4] = {16, 16, 0, 0}; };
+template <> struct Radices<512> { static constexpr int n = 3, r[4] = {8, 8, 8, 0}; };
+template <> struct Radices<1024> { static constexpr int n = 3, r[4] = {16, 8, 8, 0}; };
+template <> struct Radices<2048> { static constexpr int n = 3, r[4] = {16, 16, 8, 0}; };
+template <> struct Radices<4096> { static constexpr int n = 3, r[4] = {16, 16, 16, 0}; };
+template <> struct Radices<8192> { static constexpr int n = 4, r[4] = {16, 16, 8, 4}; };
+
+template <int N, int S>
+struct StageInfo {
+  static constexpr int R = Radices<N>::r[S];
+  static constexpr int NS = S == 0 ? 1 : StageInfo<N, S - 1>::NS * StageInfo<N, S - 1>::R;
+};
+template <int N>
+struct StageInfo<N, -1> {
+  static constexpr int R = 1, NS = 1;
+};
+
+__device__ __forceinline__ int lp(int k) { return k + (k >> 4); }
+
+template <typename R>
+struct V2;
+template <>
+struct V2<float> {
+  typedef float t __attribute__((ext_vector_type(2)));
+};
+template <>
+struct V2<double> {
+  typedef double t __attribute__((ext_vector_type(2)));
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> ld(const cpx<R>* p, bool nt) {
+  typedef typename V2<R>::t V;
+  V v = nt ? __builtin_nontemporal_load(reinterpret_cast<const V*>(p)) : *reinterpret_cast<const V*>(p);
+  return {v.x, v.y};
+}
+template <typename R>
+__device__ __forceinline__ void st(cpx<R>* p, cpx<R> x, bool nt) {
+  typedef typename V2<R>::t V;
+  V v;
+  v.x = x.x;
+  v.y = x.y;
+  if (nt)
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+  else
+    *reinterpret_cast<V*>(p) = v;
+}
+
+__device__ __forceinline__ uint32_t fdv(uint32_t x, sa_fastdiv_t f) {
+  return fdiv(x, FastDiv{f.d, f.m, f.s});
+}
+
+struct Line {
+  int64_t in, out;
+  uint64_t dig;  // four-step twiddle digit
+};
+
+__device__ __forceinline__ Line line_of(const sa_pass_t& a, uint32_t b) {
+  const uint32_t q2 = fdv(b, a.n2div);
+  const uint32_t d2 = b - q2 * (uint32_t)a.n2;
+  const uint32_t d0 = fdv(q2, a.n1div);
+  const uint32_t d1 = q2 - d0 * (uint32_t)a.n1;
+  Line l;
+  l.in = (int64_t)d0 * a.is0 + (int64_t)d1 * a.is1 + (int64_t)d2 * a.is2;
+  l.out = (int64_t)d0 * a.os0 + (int64_t)d1 * a.os1 + (int64_t)d2 * a.os2;
+  l.dig = (uint64_t)((a.ptw_level == 0) ? d0 : 0u) + (uint64_t)((a.ptw_level == 1) ? d1 : 0u) +
+          (uint64_t)((a.ptw_level == 2) ? d2 : 0u);
+  return l;
+}
+
+// Thread identity (line slot, lane in line) for a mapping kind.
+template <int T>
+__device__ __forceinline__ void ident(uint32_t t, bool kfast, int logC, uint32_t& c, uint32_t& tl) {
+  if (kfast) {
+    c = t >> ilog2(T);
+    tl = t & (T - 1);
+  } else {
+    c = t & ((1u << logC) - 1);
+    tl = t >> logC;
+  }
+}
+
+// v[m] = element tl + T*m of line b (zeros past the last line).
+template <typename R, int N>
+__device__ __forceinline__ void load_tile(const sa_pass_t& a, uint32_t b, uint32_t tl,
+                                          cpx<R> (&v)[Elems<R>::E]) {
+  constexpr int E = Elems<R>::E, T = N / E;
+  if (b < (uint32_t)a.batch) {
+    const Line l = line_of(a, b);
+    const cpx<R>* src = reinterpret_cast<const cpx<R>*>(a.in) + l.in + (int64_t)tl * a.iks;
+    const int64_t step = (int64_t)T * a.iks;
+    const bool nt = a.nt != 0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      v[m] = ld(src, nt);
+      src += step;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = {R(0), R(0)};
+  }
+}
+
+// Per-line store context: destination, element step and the four-step
+// twiddle exponent e(m) = dig * (tl + T*m) mod Nt.
+template <typename R>
+struct StoreCtx {
+  cpx<R>* dst;
+  int64_t step;
+  uint64_t e0, es, Nt;
+  bool ok;
+};
+
+template <typename R, int N>
+__device__ __forceinline__ StoreCtx<R> store_ctx(const sa_pass_t& a, uint32_t b, uint32_t tl) {
+  constexpr int T = N / Elems<R>::E;
+  StoreCtx<R> c;
+  c.ok = b < (uint32_t)a.batch;
+  const Line l = line_of(a, c.ok ? b : 0u);
+  c.dst = reinterpret_cast<cpx<R>*>(a.out) + l.out + (int64_t)tl * a.oks;
+  c.step = (int64_t)T * a.oks;
+  c.Nt = a.ptw_level >= 0 ? (uint64_t)a.ptw_N : 1;
+  c.e0 = a.ptw_level >= 0 ? (l.dig * tl) % c.Nt : 0;
+  c.es = a.ptw_level >= 0 ? (l.dig * T) % c.Nt : 0;
+  return c;
+}
+
+// Store one output x of slot m (element tl + T*m) with twiddle and scale.
+template <typename R, int SIGN>
+__device__ __forceinline__ void put(const sa_pass_t& a, const StoreCtx<R>& c, int m, uint64_t e,
+                                    cpx<R> x) {
+  if (a.ptw_level >= 0) {
+    const cpx<R>* lo = reinterpret_cast<const cpx<R>*>(a.ptw_lo);
+    const cpx<R>* hi = reinterpret_cast<const cpx<R>*>(a.ptw_hi);
+    const cpx<R> w = cmul(hi[e >> a.ptw_shift], lo[e & ((1ull << a.ptw_shift) - 1)]);
+    x = cmul_dir<SIGN>(x, w);
+  }
+  if (a.scale != 1.0) x = cscale(x, (R)a.scale);
+  st(c.dst + m * c.step, x, a.nt != 0);
+}
+
+// e0 + k*es mod Nt for small k (k*es < 2^63 guaranteed by Nt < 2^40).
+__device__ __forceinline__ uint64_t emod(uint64_t e0, uint64_t es, int k, uint64_t Nt) {
+  return (e0 + (uint64_t)k * es) % Nt;
+}
+
+// Stage 0 from registers into LDS (Ns = 1: output index jb*RAD + r).
+template <typename R, int N, int SIGN>
+__device__ __forceinline__ void stage_first(const cpx<R> (&v)[Elems<R>::E], cpx<R>* line,
+                                            uint32_t tl) {
+  constexpr int E = Elems<R>::E, RAD = Radices<N>::r[0];
+  constexpr int T = N / E, MAXB = E / RAD;
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    cpx<R> u[RAD];
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) u[r] = v[i + r * MAXB];
+    dft<RAD, SIGN>(u);
+    const int idx = (int)(tl + i * T) * RAD;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) line[lp(idx | r)] = u[r];
+  }
+}
+
+// Middle stage S: LDS -> LDS (a barrier precedes the call).
+template <typename R, int N, int S, int SIGN>
+__device__ __forceinline__ void stage_mid(cpx<R>* line, const cpx<R>* __restrict__ tw,
+                                          uint32_t tl) {
+  constexpr int E = Elems<R>::E;
+  constexpr int RAD = StageInfo<N, S>::R, NS = StageInfo<N, S>::NS;
+  constexpr int T = N / E, MAXB = E / RAD, NB = N / RAD;
+  cpx<R> u[MAXB][RAD];
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int jb = (int)tl + i * T;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) u[i][r] = line[lp(jb + r * NB)];
+    const int rem = jb & (NS - 1);
+    const cpx<R>* w = tw + rem * (RAD - 1);
+#pragma unroll
+    for (int r = 1; r < RAD; ++r) u[i][r] = cmul_dir<SIGN>(u[i][r], w[r - 1]);
+    dft<RAD, SIGN>(u[i]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int jb = (int)tl + i * T;
+    const int rem = jb & (NS - 1);
+    const int base = ((jb >> ilog2(NS)) << ilog2(NS * RAD)) | rem;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) line[lp(base + r * NS)] = u[i][r];
+  }
+}
+
+template <typename R, int N, int SIGN, int S>
+__device__ __forceinline__ void mids(cpx<R>* line, const sa_pass_t& a, uint32_t tl) {
+  if constexpr (S < Radices<N>::n - 1) {
+    __syncthreads();
+    stage_mid<R, N, S, SIGN>(line, reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[S], tl);
+    mids<R, N, SIGN, S + 1>(line, a, tl);
+  }
+}
+
+// Last stage (Ns = N/RAD): butterfly jb = tl + i*T yields output elements
+// jb + r*Ns = tl + T*(i + r*MAXB).  STORE: write them straight to HBM;
+// otherwise park them in LDS in natural order for a re-mapped store.
+template <typename R, int N, int SIGN, bool STORE>
+__device__ __forceinline__ void stage_last(const sa_pass_t& a, cpx<R>* line, uint32_t tl,
+                                           const StoreCtx<R>& c) {
+  constexpr int E = Elems<R>::E, S = Radices<N>::n - 1;
+  constexpr int RAD = StageInfo<N, S>::R;
+  constexpr int T = N / E, MAXB = E / RAD, NB = N / RAD;
+  const cpx<R>* tw = reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[S];
+  if constexpr (STORE) {
+    if (!c.ok) return;
+    const uint64_t step_r = ((uint64_t)MAXB * c.es) % c.Nt;
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+      const int jb = (int)tl + i * T;
+      cpx<R> u[RAD];
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) u[r] = line[lp(jb + r * NB)];
+      const cpx<R>* w = tw + jb * (RAD - 1);
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) u[r] = cmul_dir<SIGN>(u[r], w[r - 1]);
+      dft<RAD, SIGN>(u);
+      uint64_t e = emod(c.e0, c.es, i, c.Nt);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        put<R, SIGN>(a, c, i + r * MAXB, e, u[r]);
+        e += step_r;
+        if (e >= c.Nt) e -= c.Nt;
+      }
+    }
+  } else {
+    cpx<R> u[MAXB][RAD];
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+      const int jb = (int)tl + i * T;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) u[i][r] = line[lp(jb + r * NB)];
+      const cpx<R>* w = tw + jb * (RAD - 1);
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) u[i][r] = cmul_dir<SIGN>(u[i][r], w[r - 1]);
+      dft<RAD, SIGN>(u[i]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i)
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) line[lp((int)tl + T * (i + r * MAXB))] = u[i][r];
+  }
+}
+
+template <typename R, int N, int SIGN>
+__global__ __launch_bounds__(Elems<R>::MAXT) void fft_fixed_pass(const sa_pass_t a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(lds_raw);
+  constexpr int E = Elems<R>::E, T = N / E;
+  const uint32_t t = threadIdx.x;
+  const int logC = a.logC;
+  const int stride = a.ldsstride;
+  const uint32_t C = 1u << logC;
+  const bool same = a.ld_kfast == a.st_kfast;
+  uint32_t c, tl, sc, stl;
+  ident<T>(t, a.ld_kfast != 0, logC, c, tl);
+  ident<T>(t, a.st_kfast != 0, logC, sc, stl);
+  cpx<R>* line = lds + (size_t)c * stride;
+
+  const uint32_t tiles = (uint32_t)((a.batch + C - 1) >> logC);
+  uint32_t tile = blockIdx.x;
+  cpx<R> v[E];
+  if (tile < tiles) load_tile<R, N>(a, (tile << logC) + c, tl, v);
+  while (tile < tiles) {
+    stage_first<R, N, SIGN>(v, line, tl);
+    const uint32_t next = tile + gridDim.x;
+    if (next < tiles) load_tile<R, N>(a, (next << logC) + c, tl, v);  // prefetch
+    mids<R, N, SIGN, 1>(line, a, tl);
+    __syncthreads();
+    if (same) {
+      const StoreCtx<R> ctx = store_ctx<R, N>(a, (tile << logC) + c, tl);
+      stage_last<R, N, SIGN, true>(a, line, tl, ctx);
+    } else {
+      // Transposing pass: natural order through LDS, then the store mapping.
+      const StoreCtx<R> dummy = {};
+      stage_last<R, N, SIGN, false>(a, line, tl, dummy);
+      __syncthreads();
+      const StoreCtx<R> ctx = store_ctx<R, N>(a, (tile << logC) + sc, stl);
+      if (ctx.ok) {
+        const cpx<R>* sl = lds + (size_t)sc * stride;
+        uint64_t e = ctx.e0;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          put<R, SIGN>(a, ctx, m, e, sl[lp((int)stl + T * m)]);
+          e += ctx.es;
+          if (e >= ctx.Nt) e -= ctx.Nt;
+        }
+      }
+    }
+    __syncthreads();
+    tile = next;
+  }
+}
+
+template <typename R, int N, int SIGN>
+hipError_t launch_n(const sa_pass_t* p, hipStream_t s, int grid) {
+  const uint32_t nthr = (uint32_t)((N / Elems<R>::E) << p->logC);
+  const size_t lds = ((size_t)p->ldsstride << p->logC) * sizeof(cpx<R>);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_fixed_pass<R, N, SIGN>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL((fft_fixed_pass<R, N, SIGN>), dim3((uint32_t)grid), dim3(nthr), lds, s, *p);
+  return hipGetLastError();
+}
+
+template <typename R, int SIGN>
+hipError_t launch_r(const sa_pass_t* p, hipStream_t s, int grid) {
+  switch (p->n) {
+    case 32: return launch_n<R, 32, SIGN>(p, s, grid);
+    case 64: return launch_n<R, 64, SIGN>(p, s, grid);
+    case 128: return launch_n<R, 128, SIGN>(p, s, grid);
+    case 256: return launch_n<R, 256, SIGN>(p, s, grid);
+    case 512: return launch_n<R, 512, SIGN>(p, s, grid);
+    case 1024: return launch_n<R, 1024, SIGN>(p, s, grid);
+    case 2048: return launch_n<R, 2048, SIGN>(p, s, grid);
+    case 4096: return launch_n<R, 4096, SIGN>(p, s, grid);
+    case 8192: return launch_n<R, 8192, SIGN>(p, s, grid);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace fx
+}  // namespace sa
+
+extern "C" int sa_fixed_supported(int n) {
+  return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 ||
+         n == 4096 || n == 8192;
+}
+
+extern "C" int sa_fixed_elems(int prec) { (void)prec; return 16; }
+
+extern "C" int sa_fixed_max_threads(int prec) { return prec == 0 ? 1024 : 512; }
+
+extern "C" int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid, void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (prec == 0)
+    return sign < 0 ? (int)sa::fx::launch_r<float, -1>(p, s, grid)
+                    : (int)sa::fx::launch_r<float, +1>(p, s, grid);
+  return sign < 0 ? (int)sa::fx::launch_r<double, -1>(p, s, grid)
+                  : (int)sa::fx::launch_r<double, +1>(p, s, grid);
+}
+
+// Resident workgroups per CU for the fixed kernel of (n, prec) with the
+// given block size and LDS bytes.
+extern "C" int sa_fixed_occupancy(int n, int prec, int sign, int nthr, size_t lds) {
+  int blocks = 0;
+  const void* f = nullptr;
+#define SA_FX_CASE(NN)                                                                        \
+  case NN:                                                                                    \
+    f = prec == 0 ? (sign < 0 ? reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<float, NN, -1>)  \
+                              : reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<float, NN, 1>))   \
+                  : (sign < 0 ? reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<double, NN, -1>) \
+                              : reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<double, NN, 1>)); \
+    break;
+  switch (n) {
+    SA_FX_CASE(32)
+    SA_FX_CASE(64)
+    SA_FX_CASE(128)
+    SA_FX_CASE(256)
+    SA_FX_CASE(512)
+    SA_FX_CASE(1024)
+    SA_FX_CASE(2048)
+    SA_FX_CASE(4096)
+    SA_FX_CASE(8192)
+    default: return 0;
+  }
+#undef SA_FX_CASE
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, nthr, lds) != hipSuccess) return 0;
+  return blocks;
+}

@@ -14,6 +14,7 @@
 #ifndef SHAFFT_AMD_PASS_DESC_H
 #define SHAFFT_AMD_PASS_DESC_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -48,7 +49,11 @@ typedef struct {
   int32_t T;         /* threads per line                                 */
   int32_t ld_kfast;  /* 1: lanes walk k on load, 0: lanes walk lines     */
   int32_t st_kfast;  /* same for the store                               */
-  int32_t ldsstride; /* complex elements per line slot in LDS            */
+  int32_t ldsstride; /* complex elements per line slot in LDS (padded)   */
+  int32_t reg_first; /* stage 0 straight from the loaded registers       */
+  int32_t reg_last;  /* last stage straight into the stored registers    */
+  int32_t nt;        /* non-temporal global loads/stores of the data     */
+  int32_t logC;      /* log2(C) for the fixed-length kernel (C a power of 2) */
   sa_fastdiv_t Tdiv, Cdiv, n1div, n2div;
 
   /* Optional post-twiddle on the store: out *= W_N^{+-(digit * k) mod N}
@@ -83,6 +88,16 @@ typedef struct {
 /* Launch one pass.  prec: 0 = complex float, 1 = complex double.
  * sign: -1 forward, +1 backward.  Returns a hipError_t value. */
 int sa_launch_pass(const sa_pass_t* p, int prec, int sign, void* stream);
+
+/* Fixed-length persistent kernel (power-of-two n in [32, 8192], no load or
+ * store hooks other than the four-step twiddle and scale).  `grid` is the
+ * number of persistent workgroups. */
+int sa_fixed_supported(int n);
+int sa_fixed_elems(int prec); /* elements per thread (N / threads per line) */
+int sa_fixed_max_threads(int prec); /* workgroup size bound */
+int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid, void* stream);
+/* Resident workgroups per CU of that kernel for a block size and LDS size. */
+int sa_fixed_occupancy(int n, int prec, int sign, int nthr, size_t lds);
 
 /* x[i] *= f for i < count complex elements. */
 int sa_launch_scale(void* data, double f, uint64_t count, int prec, void* stream);

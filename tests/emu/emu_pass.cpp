@@ -12,6 +12,7 @@
 #include <array>
 #include <cstdio>
 #include <map>
+#include <vector>
 #include <cstring>
 #include <algorithm>
 #include <vector>
@@ -85,6 +86,8 @@ void stages(const sa_pass_t& p, std::vector<C<R>>& d, int sign) {
   }
 }
 
+thread_local bool g_fixed_launch = false;
+
 template <typename R>
 void check_shape(const sa_pass_t& p) {
   auto die = [](const char* m) {
@@ -92,14 +95,23 @@ void check_shape(const sa_pass_t& p) {
     std::abort();
   };
   const int emax = 16;
-  if (p.T < 1 || p.C < 1 || p.C * p.T > 512) die("threads per workgroup");
+  if (p.T < 1 || p.C < 1 || p.C * p.T > (g_fixed_launch ? 1024 : 512)) die("threads per workgroup");
   if ((int64_t)p.n > (int64_t)emax * p.T) die("n > EMAX * T");
   for (int s = 0; s < p.nstages; ++s) {
     const int rad = p.radix[s];
     const int maxb = (emax + rad - 1) / rad;
-    if ((p.n / rad + p.T - 1) / p.T > maxb) die("butterflies per thread exceed register budget");
+    if (!g_fixed_launch && (p.n / rad + p.T - 1) / p.T > maxb)
+      die("butterflies per thread exceed register budget");
   }
-  if (p.ldsstride < p.n) die("lds stride");
+  if (p.ldsstride < p.n + (p.n - 1) / 16) die("lds stride");
+  if (p.reg_first || p.reg_last) {
+    if (p.n != emax * p.T || p.nstages < 2 || p.blu_h) die("register stage on an unsuitable pass");
+    auto p2 = [](int r) { return r == 2 || r == 4 || r == 8 || r == 16; };
+    if (p.reg_first && !p2(p.radix[0])) die("reg_first radix");
+    if (p.reg_last && !p2(p.radix[p.nstages - 1])) die("reg_last radix");
+    const int id_kfast = p.reg_first ? p.ld_kfast : p.st_kfast;
+    if (p.reg_last && id_kfast != p.st_kfast) die("reg_last with a different store mapping");
+  }
   if ((int64_t)p.C * p.ldsstride * (int64_t)sizeof(C<R>) > 160 * 1024) die("lds bytes");
   if (p.batch >= (int64_t(1) << 31)) die("batch");
   if (p.ptw_level > 2 || p.ld_level > 2 || p.st_level > 2 || p.ld_level2 > 2 || p.st_level2 > 2)
@@ -196,6 +208,48 @@ extern "C" int sa_launch_pass(const sa_pass_t* p, int prec, int sign, void*) {
   else
     run_pass<double>(*p, sign);
   return 0;
+}
+
+extern "C" int sa_fixed_supported(int n) {
+  return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 ||
+         n == 4096 || n == 8192;
+}
+
+extern "C" int sa_fixed_occupancy(int, int, int, int, size_t) { return 1; }
+extern "C" int sa_fixed_elems(int) { return 16; }
+extern "C" int sa_fixed_max_threads(int prec) { return prec == 0 ? 1024 : 512; }
+
+// The fixed kernel has the same semantics; check its own launch shape first.
+extern "C" int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid, void*) {
+  auto die = [](const char* m) {
+    std::fprintf(stderr, "EMU BAD FIXED LAUNCH: %s\n", m);
+    std::abort();
+  };
+  if (!sa_fixed_supported(p->n)) die("length");
+  if (p->T * sa_fixed_elems(prec) != p->n) die("T");
+  if (p->C != (1 << p->logC)) die("C");
+  if (p->C * p->T > sa_fixed_max_threads(prec)) die("threads");
+  if ((int64_t)p->C * p->ldsstride * (prec ? 16 : 8) > 160 * 1024) die("lds");
+  if (p->ldsstride < p->n + (p->n - 1) / 16) die("stride");
+  if (grid < 1) die("grid");
+  if (p->pre || p->post || p->blu_h || p->ld_level >= 0 || p->st_level >= 0) die("hooks");
+  {
+    // the kernel's compile-time radix sequence (fft_fixed.hip Radices<N>)
+    static const std::map<int, std::vector<int>> plans = {
+        {32, {8, 4}},       {64, {8, 8}},       {128, {16, 8}},     {256, {16, 16}},
+        {512, {8, 8, 8}},   {1024, {16, 8, 8}}, {2048, {16, 16, 8}}, {4096, {16, 16, 16}},
+        {8192, {16, 16, 8, 4}}};
+    const auto& pl = plans.at(p->n);
+    if ((int)pl.size() != p->nstages) die("stage count differs from the kernel's plan");
+    for (int s = 0; s < p->nstages; ++s)
+      if (pl[s] != p->radix[s]) die("radix differs from the kernel's plan");
+  }
+  sa_pass_t q = *p;
+  q.reg_first = q.reg_last = 0;
+  g_fixed_launch = true;
+  const int rc = sa_launch_pass(&q, prec, sign, nullptr);
+  g_fixed_launch = false;
+  return rc;
 }
 
 extern "C" int sa_launch_scale(void* data, double f, uint64_t count, int prec, void*) {
