@@ -7,7 +7,7 @@ the reference-style C API (each call host-synchronous, as in SHAFFT).
 GFLOP/s counts 5 N log2 N per direction (2 directions per step).
 
 Also reported, on the same JSON line:
-  * roofline: the dominant kernel (the forward LDS pass kernel), its average
+  * roofline: the dominant kernel (the forward pass kernel), its average
     launch duration from HIP events recorded on the plan's stream around the
     forward executes of the timed region, and its algorithmic bytes per
     launch (one read + one write of the 2^27 complex128 grid = 4.295 GB);
@@ -15,6 +15,7 @@ Also reported, on the same JSON line:
     host on one forward transform of the same grid.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                       [--dims 512,512,512] [--prec z2z|c2c] [--calibrate-copy]
 """
 import argparse
 import ctypes as ct
@@ -70,22 +71,39 @@ class Hip:
         assert self.L.hipDeviceSynchronize() == 0
 
 
-def cpu_baseline(x, dims, prec):
+def cpu_baseline(x, dims, prec, budget_s=10.0):
     import oracle  # the CPU-baseline leg is the only bench use of the oracle
 
     a = x.reshape(dims).astype(np.complex128)
-    t0 = time.perf_counter()
-    oracle.fftn_pow2(a, None, -1)
-    dt = time.perf_counter() - t0
+    reps, t_total = 0, 0.0
+    while reps < 3 or (t_total < budget_s and reps < 50):
+        t0 = time.perf_counter()
+        oracle.fftn_pow2(a, None, -1)
+        t_total += time.perf_counter() - t0
+        reps += 1
     n = a.size
     return {
-        "value": round(5 * n * math.log2(n) / dt / 1e9, 3),
+        "value": round(reps * 5 * n * math.log2(n) / t_total / 1e9, 3),
         "unit": "GFLOP/s",
         "cores": oracle.threads(),
         "kind": "port",
-        "sample": f"one forward FFTND of the full {'x'.join(map(str, dims))} grid in FP64 "
-                  f"({dt:.2f} s), oracle radix-2 FFT with OpenMP over lines; FFTW is not installed",
+        "sample": f"{reps} forward FFTND transforms of the full {'x'.join(map(str, dims))} grid in FP64 "
+                  f"({t_total:.1f} s), oracle radix-2 FFT with OpenMP over lines; FFTW is not installed",
     }
+
+
+def profiled_traffic(dims, prec):
+    """HBM bytes per launch of the forward pass kernel measured with rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE (scaled by a known-size copy), read from the
+    committed profile summary when it covers this workload."""
+    name = f"fft_fixed_pass_{'x'.join(map(str, dims))}_{'z2z' if prec else 'c2c'}.md"
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def main():
@@ -96,6 +114,9 @@ def main():
     ap.add_argument("--dims", type=str, default="512,512,512")
     ap.add_argument("--prec", type=str, default="z2z", choices=["c2c", "z2z"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one device-to-device copy of the data buffer first (known-size "
+                         "reference kernel for HBM counter scaling under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU per process; multi-GPU runs "
@@ -129,6 +150,12 @@ def main():
     npass = seam.num_passes()
     pass_bytes = [seam.pass_bytes(i) for i in range(npass)]
     del seam
+
+    if args.calibrate_copy:
+        hip.L.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+        assert hip.L.hipMemcpy(ct.c_void_p(b.ptr), ct.c_void_p(a.ptr), ct.c_size_t(n * esz), 3) == 0
+        hip.sync()
+        print(f"calibration copy: {n * esz} bytes read, {n * esz} bytes written", file=sys.stderr)
 
     for _ in range(args.warmup):
         plan.execute(api.FORWARD)
@@ -177,6 +204,8 @@ def main():
             "forward_ms": round(float(np.mean(fwd_ms)), 4),
             "forward_gflops": round(5 * n * math.log2(n) / (np.mean(fwd_ms) * 1e-3) / 1e9, 2),
             "step_hbm_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "forward_hbm_frac": round(sum(pass_bytes) / (np.mean(fwd_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pass_bytes": pass_bytes,
         },
         "roofline": {
             "bound": "hbm",
@@ -184,8 +213,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "fft_lds_pass (forward)",
+            "traffic": profiled_traffic(dims, prec),
+            "kernel": f"fft_fixed_pass<{'double' if prec == api.Z2Z else 'float'},{dims[-1]},-1> (forward)",
             "bytes_per_launch": int(bytes_launch),
             "avg_launch_ms": round(avg_launch_ms, 4),
         },
