@@ -102,6 +102,30 @@ int shafftAmd1DPassBytes(shafft_amd_1d_plan* plan, int pass, int64_t* bytes);
 int shafftAmdNDDescribe(shafft_amd_nd_plan* plan, char* buf, int buflen);
 int shafftAmd1DDescribe(shafft_amd_1d_plan* plan, char* buf, int buflen);
 
+/* Batched uneven transforms (no reference counterpart: the reference runs
+ * one plan per shape).  `count` independent transforms, transform t having
+ * ndims[t] axes taken in order from dims (row-major, last axis fastest),
+ * packed back to back (offsets from shafftAmdBatchLayout) in one buffer of
+ * one precision.  Every axis must fit one LDS pass (length <= 8192 complex
+ * float / 4096 complex double, or a Bluestein length whose padded
+ * convolution does).  All passes of all transforms run as one persistent
+ * launch; execute replays a captured HIP graph per direction. */
+typedef struct shafft_amd_batch_plan shafft_amd_batch_plan;
+int shafftAmdBatchPlan(shafft_amd_batch_plan** plan, int count, const int* ndims,
+                       const int* dims, int precision);
+int shafftAmdBatchLayout(shafft_amd_batch_plan* plan, int64_t* total_elems, int64_t* offsets);
+int shafftAmdBatchSetBuffers(shafft_amd_batch_plan* plan, void* in, void* out, void* work);
+int shafftAmdBatchSetStream(shafft_amd_batch_plan* plan, void* stream);
+/* Asynchronous on the plan's stream; use_graph = 0 launches directly. */
+int shafftAmdBatchExecute(shafft_amd_batch_plan* plan, int direction, int use_graph);
+int shafftAmdBatchSynchronize(shafft_amd_batch_plan* plan);
+/* Times the candidate tile schedules on the attached buffers (overwrites
+ * out and work), keeps the fastest and writes a report into buf. */
+int shafftAmdBatchAutotune(shafft_amd_batch_plan* plan, char* buf, int buflen);
+int shafftAmdBatchInfo(shafft_amd_batch_plan* plan, int* npasses, int64_t* nitems, int* grid,
+                       int64_t* bytes);
+int shafftAmdBatchDestroy(shafft_amd_batch_plan* plan);
+
 /* ======================================================================== */
 /* 2. Reference C API (include/shafft/shafft.h) at one rank per GPU          */
 /* ======================================================================== */

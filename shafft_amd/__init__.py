@@ -10,6 +10,7 @@ missing, importing :func:`lib` raises.
 
 from .api import (  # noqa: F401
     BACKWARD,
+    BatchPlan,
     C2C,
     FORWARD,
     Z2Z,

@@ -58,6 +58,17 @@ _SIGS = {
     "shafftAmd1DPassBytes": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64)]),
     "shafftAmdNDDescribe": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_int]),
     "shafftAmd1DDescribe": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_int]),
+    "shafftAmdBatchPlan": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                                      ct.c_int]),
+    "shafftAmdBatchLayout": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
+    "shafftAmdBatchSetBuffers": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "shafftAmdBatchSetStream": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "shafftAmdBatchExecute": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int]),
+    "shafftAmdBatchSynchronize": (ct.c_int, [ct.c_void_p]),
+    "shafftAmdBatchAutotune": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_int]),
+    "shafftAmdBatchInfo": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int),
+                                      ct.POINTER(ct.c_int64)]),
+    "shafftAmdBatchDestroy": (ct.c_int, [ct.c_void_p]),
     # reference C API
     "shafftSetStream": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "shafftLastErrorStatus": (ct.c_int, []),
@@ -365,6 +376,59 @@ class OneDBackendPlan:
         try:
             if self.h.value:
                 self.L.shafftAmd1DDestroy(self.h)
+                self.h = ct.c_void_p()
+        except Exception:
+            pass
+
+
+class BatchPlan:
+    """Many transforms of uneven shape in one persistent launch
+    (shafftAmdBatch* in include/shafft_amd.h)."""
+
+    def __init__(self, shapes, prec: int, L=None):
+        self.L = L or _L()
+        self.h = ct.c_void_p()
+        self.shapes = [tuple(int(v) for v in sh) for sh in shapes]
+        nd = (ct.c_int * len(self.shapes))(*[len(sh) for sh in self.shapes])
+        flat = [v for sh in self.shapes for v in sh]
+        dd = (ct.c_int * len(flat))(*flat)
+        check(self.L.shafftAmdBatchPlan(ct.byref(self.h), len(self.shapes), nd, dd, prec), "shafftAmdBatchPlan", self.L)
+        self.prec = prec
+        tot = ct.c_int64()
+        offs = (ct.c_int64 * len(self.shapes))()
+        check(self.L.shafftAmdBatchLayout(self.h, ct.byref(tot), offs), "shafftAmdBatchLayout", self.L)
+        self.total = tot.value
+        self.offsets = list(offs)
+
+    def set_buffers(self, src: int, dst: int, work: int):
+        check(self.L.shafftAmdBatchSetBuffers(self.h, ct.c_void_p(src), ct.c_void_p(dst), ct.c_void_p(work)),
+              "shafftAmdBatchSetBuffers", self.L)
+
+    def set_stream(self, stream: int):
+        check(self.L.shafftAmdBatchSetStream(self.h, ct.c_void_p(stream)), "shafftAmdBatchSetStream", self.L)
+
+    def execute(self, direction: int, use_graph: bool = True):
+        check(self.L.shafftAmdBatchExecute(self.h, direction, 1 if use_graph else 0), "shafftAmdBatchExecute", self.L)
+
+    def synchronize(self):
+        check(self.L.shafftAmdBatchSynchronize(self.h), "shafftAmdBatchSynchronize", self.L)
+
+    def autotune(self) -> str:
+        buf = ct.create_string_buffer(4096)
+        check(self.L.shafftAmdBatchAutotune(self.h, buf, 4096), "shafftAmdBatchAutotune", self.L)
+        return buf.value.decode()
+
+    def info(self):
+        np_, grid = ct.c_int(), ct.c_int()
+        items, nbytes = ct.c_int64(), ct.c_int64()
+        check(self.L.shafftAmdBatchInfo(self.h, ct.byref(np_), ct.byref(items), ct.byref(grid), ct.byref(nbytes)),
+              "shafftAmdBatchInfo", self.L)
+        return {"passes": np_.value, "items": items.value, "grid": grid.value, "bytes": nbytes.value}
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                self.L.shafftAmdBatchDestroy(self.h)
                 self.h = ct.c_void_p()
         except Exception:
             pass

@@ -1,0 +1,267 @@
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <sstream>
+
+#include "fastdiv.hpp"
+
+namespace sa {
+
+static int cu_count() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 0;
+    cached = 256;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      cached = v;
+  }
+  return cached;
+}
+
+BatchPlan::BatchPlan(int count, const int* ndims, const int* dims_flat, int prec)
+    : prec_(prec), prog_(prec) {
+  if (prec != 0 && prec != 1) fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
+  if (count < 1 || !ndims || !dims_flat) fail(ERR_INVALID_DIM, "empty batch");
+  const int64_t cap = lds_cap(prec);
+  int64_t off = 0;
+  const int* d = dims_flat;
+  for (int t = 0; t < count; ++t) {
+    const int nd = ndims[t];
+    if (nd < 1) fail(ERR_INVALID_DIM, "transform with no axes");
+    int64_t elems = 1;
+    for (int i = 0; i < nd; ++i) {
+      if (d[i] < 1) fail(ERR_INVALID_DIM, "dimension must be positive");
+      elems *= d[i];
+    }
+    Tr tr{off, elems, (int)prog_.npasses(), 0};
+    for (int ax = nd - 1; ax >= 0; --ax) {
+      const int64_t n = d[ax];
+      if (n == 1) continue;
+      // One LDS pass per axis: a length that needs a multi-pass split
+      // would need scratch of its own and is left to the per-call plans.
+      const bool fits = (n <= cap && is_smooth7(n));
+      int64_t m = 1;
+      while (m < 2 * n - 1) m <<= 1;
+      if (!fits && m > cap) fail(ERR_NOT_IMPL, "batched axis longer than one LDS pass");
+      int64_t outer = 1, inner = 1;
+      for (int i = 0; i < ax; ++i) outer *= d[i];
+      for (int i = ax + 1; i < nd; ++i) inner *= d[i];
+      std::vector<Digit> lines = {{outer, n * inner, n * inner}, {inner, 1, 1}};
+      prog_.add_transform(n, lines, inner, inner);
+    }
+    tr.np = (int)prog_.npasses() - tr.first;
+    tr_.push_back(tr);
+    off += elems;
+    d += nd;
+  }
+  total_ = off;
+}
+
+BatchPlan::~BatchPlan() { drop_graphs(); }
+
+void BatchPlan::drop_graphs() {
+  for (int i = 0; i < 2; ++i) {
+    if (graph_[i]) (void)hipGraphExecDestroy(graph_[i]);
+    if (gdef_[i]) (void)hipGraphDestroy(gdef_[i]);
+    graph_[i] = nullptr;
+    gdef_[i] = nullptr;
+  }
+}
+
+void BatchPlan::set_buffers(void* in, void* out, void* work) {
+  if (!in || !out || !work) fail(ERR_NULLPTR, "batch buffers must be set");
+  if (work == in || work == out) fail(ERR_INVALID_STATE, "work buffer must be distinct");
+  in_ = in;
+  out_ = out;
+  work_ = work;
+  drop_graphs();
+  build();
+}
+
+void BatchPlan::build() {
+  const int esz = prec_ ? 16 : 8;
+  const size_t np = prog_.npasses();
+  std::vector<sa_pass_t> desc[2];
+  std::vector<uint32_t> tiles(np);
+  std::vector<int32_t> dep(np, -1);
+  std::vector<int8_t> sign[2];
+  desc[0].resize(np);
+  desc[1].resize(np);
+  sign[0].resize(np);
+  sign[1].resize(np);
+  block_ = 1;
+  lds_ = 0;
+  const int budget = 160 * 1024;
+  for (const Tr& tr : tr_) {
+    for (int j = 0; j < tr.np; ++j) {
+      const size_t i = (size_t)(tr.first + j);
+      const PassSpec& ps = prog_.pass(i);
+      sa_pass_t p = ps.p;
+      // Re-tile for the batch kernel (generic LDS path).
+      const int n = p.n;
+      const int T = (n + 15) / 16;
+      int C = std::max(1, sched_.block / T);
+      int stride = n + (n - 1) / 16;
+      if (stride % 2 == 0) stride += 1;
+      while (C > 1 && (int64_t)C * stride * esz > budget) --C;
+      if (C > p.batch) C = (int)std::max<int64_t>(1, p.batch);
+      p.T = T;
+      p.C = C;
+      p.ldsstride = stride;
+      p.Tdiv = make_fastdiv((uint32_t)T);
+      p.Cdiv = make_fastdiv((uint32_t)C);
+      p.reg_first = p.reg_last = 0;
+      p.nt = 0;
+      p.logC = 0;
+      block_ = std::max(block_, C * T);
+      lds_ = std::max(lds_, (size_t)C * stride * esz);
+      tiles[i] = (uint32_t)((p.batch + C - 1) / C);
+      if (j > 0) dep[i] = (int32_t)(i - 1);
+      // Buffers: pass j writes `out` when (np-1-j) is even, else `work`.
+      auto buf_of = [&](int jj) -> char* {
+        return static_cast<char*>(((tr.np - 1 - jj) % 2 == 0) ? out_ : work_);
+      };
+      const char* src = j == 0 ? static_cast<const char*>(in_) : buf_of(j - 1);
+      char* dst = buf_of(j);
+      p.in = src + tr.off * esz;
+      p.out = dst + tr.off * esz;
+      for (int d = 0; d < 2; ++d) {
+        sa_pass_t q = p;
+        if (ps.dir_tables) {
+          q.pre = ps.pre_t[d];
+          q.post = ps.post_t[d];
+          q.blu_h = ps.blu_t[d];
+        }
+        desc[d][i] = q;
+        sign[d][i] = (int8_t)(ps.sign == 0 ? (d == 0 ? -1 : +1) : ps.sign);
+      }
+    }
+  }
+  // Level-major work list.
+  items_.clear();
+  int maxnp = 0;
+  for (const Tr& tr : tr_) maxnp = std::max(maxnp, tr.np);
+  for (int lvl = 0; lvl < maxnp; ++lvl)
+    for (const Tr& tr : tr_)
+      if (lvl < tr.np) {
+        const uint32_t pi = (uint32_t)(tr.first + lvl);
+        for (uint32_t k = 0; k < tiles[pi]; ++k) items_.push_back({pi, k});
+      }
+  const int occ = std::max(1, sa_batch_occupancy(prec_, block_, lds_));
+  grid_ = (int)std::min<int64_t>((int64_t)items_.size(),
+                                 (int64_t)sched_.oversub * occ * cu_count());
+  if (grid_ < 1) grid_ = 1;
+
+  auto up = [](DevBuf& b, const void* h, size_t bytes) {
+    if (b.bytes < bytes || !b.p) b = DevBuf(std::max<size_t>(bytes, 8));
+    if (bytes) hip_check(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice), "hipMemcpy(batch)");
+  };
+  for (int d = 0; d < 2; ++d) {
+    up(d_pass_[d], desc[d].data(), np * sizeof(sa_pass_t));
+    up(d_sign_[d], sign[d].data(), np);
+  }
+  up(d_tiles_, tiles.data(), np * sizeof(uint32_t));
+  up(d_dep_, dep.data(), np * sizeof(int32_t));
+  up(d_items_, items_.data(), items_.size() * sizeof(sa_batch_item_t));
+  if (d_done_.bytes < std::max<size_t>(np, 1) * 4) d_done_ = DevBuf(std::max<size_t>(np, 1) * 4);
+  if (!d_head_.p) d_head_ = DevBuf(256);
+  built_ = true;
+}
+
+void BatchPlan::copy_identities(hipStream_t s) {
+  const int esz = prec_ ? 16 : 8;
+  if (in_ == out_) return;
+  for (const Tr& tr : tr_)
+    if (tr.np == 0)
+      hip_check(hipMemcpyAsync(static_cast<char*>(out_) + tr.off * esz,
+                               static_cast<const char*>(in_) + tr.off * esz, tr.elems * esz,
+                               hipMemcpyDeviceToDevice, s),
+                "hipMemcpyAsync(batch identity)");
+}
+
+void BatchPlan::launch(int dir, hipStream_t s) {
+  const int d = dir < 0 ? 0 : 1;
+  copy_identities(s);
+  if (items_.empty()) return;
+  sa_batch_t b{};
+  b.passes = static_cast<const sa_pass_t*>(d_pass_[d].p);
+  b.pass_tiles = static_cast<const uint32_t*>(d_tiles_.p);
+  b.pass_dep = static_cast<const int32_t*>(d_dep_.p);
+  b.pass_sign = static_cast<const int8_t*>(d_sign_[d].p);
+  b.done = static_cast<uint32_t*>(d_done_.p);
+  b.head = static_cast<uint32_t*>(d_head_.p);
+  b.items = static_cast<const sa_batch_item_t*>(d_items_.p);
+  b.nitems = (uint32_t)items_.size();
+  b.npasses = (uint32_t)prog_.npasses();
+  hip_check((hipError_t)sa_launch_batch(&b, prec_, grid_, block_, lds_, s), "batch launch");
+}
+
+void BatchPlan::execute(int dir, bool use_graph) {
+  if (!built_) fail(ERR_NO_BUFFER, "batch buffers not set");
+  if (!use_graph) {
+    launch(dir, stream_);
+    return;
+  }
+  const int d = dir < 0 ? 0 : 1;
+  if (!graph_[d]) {
+    hipStream_t cap = nullptr;
+    hip_check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      launch(dir, cap);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(cap, &g);
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipStreamDestroy(cap);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(cap, &gdef_[d]), "hipStreamEndCapture");
+    (void)hipStreamDestroy(cap);
+    hip_check(hipGraphInstantiate(&graph_[d], gdef_[d], nullptr, nullptr, 0), "hipGraphInstantiate");
+  }
+  hip_check(hipGraphLaunch(graph_[d], stream_), "hipGraphLaunch");
+}
+
+int64_t BatchPlan::total_bytes() const { return prog_.total_bytes(); }
+
+std::string BatchPlan::autotune() {
+  if (!built_) fail(ERR_NO_BUFFER, "batch buffers not set");
+  const BatchSchedule cands[] = {{256, 1}, {256, 2}, {512, 1}, {128, 2}};
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  float best = 1e30f;
+  BatchSchedule best_s = sched_;
+  std::ostringstream os;
+  for (const BatchSchedule& c : cands) {
+    sched_ = c;
+    drop_graphs();
+    build();
+    launch(-1, stream_);
+    hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
+    for (int r = 0; r < 3; ++r) launch(-1, stream_);
+    hip_check(hipEventRecord(e1, stream_), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    ms /= 3;
+    os << "block=" << c.block << " oversub=" << c.oversub << " grid=" << grid_ << " : " << ms
+       << " ms\n";
+    if (ms < best) {
+      best = ms;
+      best_s = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  sched_ = best_s;
+  drop_graphs();
+  build();
+  os << "chosen block=" << sched_.block << " oversub=" << sched_.oversub << "\n";
+  return os.str();
+}
+
+}  // namespace sa

@@ -16,6 +16,7 @@
 
 #include "../../../include/shafft_amd.h"
 #include "backend.hpp"
+#include "batch.hpp"
 #include "common.hpp"
 
 using namespace sa;
@@ -29,6 +30,9 @@ struct shafft_amd_nd_plan {
 };
 struct shafft_amd_1d_plan {
   std::unique_ptr<OneDBackend> be;
+};
+struct shafft_amd_batch_plan {
+  std::unique_ptr<BatchPlan> be;
 };
 
 namespace {
@@ -281,6 +285,88 @@ int shafftAmd1DDescribe(shafft_amd_1d_plan* plan, char* buf, int buflen) {
   return guarded([&] {
     if (!plan) fail(ERR_NULLPTR, "null plan");
     return copy_desc(plan->be->program().describe(), buf, buflen);
+  });
+}
+
+int shafftAmdBatchPlan(shafft_amd_batch_plan** plan, int count, const int* ndims,
+                       const int* dims, int precision) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan pointer");
+    *plan = nullptr;
+    auto p = std::make_unique<shafft_amd_batch_plan>();
+    p->be = std::make_unique<BatchPlan>(count, ndims, dims, precision);
+    *plan = p.release();
+    return 0;
+  });
+}
+
+int shafftAmdBatchLayout(shafft_amd_batch_plan* plan, int64_t* total_elems, int64_t* offsets) {
+  return guarded([&] {
+    if (!plan || !total_elems) fail(ERR_NULLPTR, "null argument");
+    *total_elems = plan->be->total_elems();
+    if (offsets)
+      for (int t = 0; t < plan->be->count(); ++t) offsets[t] = plan->be->offset(t);
+    return 0;
+  });
+}
+
+int shafftAmdBatchSetBuffers(shafft_amd_batch_plan* plan, void* in, void* out, void* work) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    plan->be->set_buffers(in, out, work);
+    return 0;
+  });
+}
+
+int shafftAmdBatchSetStream(shafft_amd_batch_plan* plan, void* stream) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    plan->be->set_stream(reinterpret_cast<hipStream_t>(stream));
+    return 0;
+  });
+}
+
+int shafftAmdBatchExecute(shafft_amd_batch_plan* plan, int direction, int use_graph) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    plan->be->execute(direction == 0 ? -1 : +1, use_graph != 0);
+    return 0;
+  });
+}
+
+int shafftAmdBatchSynchronize(shafft_amd_batch_plan* plan) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    hip_check(hipStreamSynchronize(plan->be->stream()), "hipStreamSynchronize");
+    return 0;
+  });
+}
+
+int shafftAmdBatchAutotune(shafft_amd_batch_plan* plan, char* buf, int buflen) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    const std::string r = plan->be->autotune();
+    if (buf && buflen > 0) copy_desc(r, buf, buflen);
+    return 0;
+  });
+}
+
+int shafftAmdBatchInfo(shafft_amd_batch_plan* plan, int* npasses, int64_t* nitems, int* grid,
+                       int64_t* bytes) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    if (npasses) *npasses = plan->be->npasses();
+    if (nitems) *nitems = plan->be->nitems();
+    if (grid) *grid = plan->be->grid();
+    if (bytes) *bytes = plan->be->total_bytes();
+    return 0;
+  });
+}
+
+int shafftAmdBatchDestroy(shafft_amd_batch_plan* plan) {
+  return guarded([&] {
+    delete plan;
+    return 0;
   });
 }
 

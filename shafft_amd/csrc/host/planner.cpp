@@ -140,7 +140,15 @@ const void* PassProgram::upload(const std::vector<cld>& t) {
 }
 
 const void* PassProgram::stage_table(const std::vector<int>& radices, std::vector<int>& offsets) {
-  return upload(stage_twiddles(radices, offsets));
+  // One table per radix sequence, shared by every pass that uses it.
+  auto it = stage_cache_.find(radices);
+  if (it != stage_cache_.end()) {
+    offsets = it->second.second;
+    return it->second.first;
+  }
+  const void* t = upload(stage_twiddles(radices, offsets));
+  stage_cache_[radices] = {t, offsets};
+  return t;
 }
 
 // Merge adjacent digits that address memory as one digit on both sides and

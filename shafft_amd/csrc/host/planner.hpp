@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <complex>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -114,6 +115,7 @@ class PassProgram {
                hipStream_t stream) const;
 
   size_t npasses() const { return passes_.size(); }
+  PassSpec& pass_mut(size_t i) { return passes_[i]; }
   const PassSpec& pass(size_t i) const { return passes_[i]; }
   int64_t total_bytes() const;
   std::string describe() const;
@@ -123,6 +125,7 @@ class PassProgram {
   int prec_;
   std::vector<PassSpec> passes_;
   std::vector<DevBuf> tables_;
+  std::map<std::vector<int>, std::pair<const void*, std::vector<int>>> stage_cache_;
 
   const void* upload(const std::vector<std::complex<long double>>& t);
   const void* stage_table(const std::vector<int>& radices, std::vector<int>& offsets);

@@ -1,0 +1,93 @@
+// Persistent batched FFT kernel for gfx950: many transforms of uneven shape
+// in one launch.
+//
+// The host flattens every pass of every transform into a work list of
+// (pass, tile) items in level-major order (all first passes, then all second
+// passes, ...).  Workgroups pull items from a device-side queue head with
+// one atomic per item.  An item whose pass depends on an earlier pass of
+// the same transform waits (one lane spins on an agent-scope acquire load)
+// until that pass's completed-tile counter is full; a finished tile makes
+// its stores visible with an agent-scope release before counting itself.
+// Every dependency of an item was handed out before the item itself, to a
+// workgroup that is already running, so the queue always drains.
+#include "fft_device.hpp"
+#include "pass_desc.h"
+#include "pass_tile.hpp"
+
+namespace sa {
+
+__global__ void batch_clear(uint32_t* done, uint32_t n, uint32_t* head) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) done[i] = 0u;
+  if (i == 0) *head = 0u;
+}
+
+template <typename R>
+__global__ __launch_bounds__(512) void fft_batch_kernel(const sa_batch_t b) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ uint32_t s_item;
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(lds_raw);
+  for (;;) {
+    if (threadIdx.x == 0)
+      s_item = __hip_atomic_fetch_add(b.head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t it = s_item;
+    __syncthreads();
+    if (it >= b.nitems) break;
+    const sa_batch_item_t w = b.items[it];
+    const int dep = b.pass_dep[w.pass];
+    if (dep >= 0) {
+      if (threadIdx.x == 0) {
+        const uint32_t need = b.pass_tiles[dep];
+        while (__hip_atomic_load(&b.done[dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
+          __builtin_amdgcn_s_sleep(1);
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    const sa_pass_t& a = b.passes[w.pass];
+    if (a.blu_h)
+      pass_tile<R, -1, MODE_BLUE>(a, w.tile, lds);
+    else if (b.pass_sign[w.pass] < 0)
+      pass_tile<R, -1, MODE_LDS>(a, w.tile, lds);
+    else
+      pass_tile<R, +1, MODE_LDS>(a, w.tile, lds);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(&b.done[w.pass], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+}  // namespace sa
+
+extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int block, size_t lds,
+                               void* stream) {
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const uint32_t n = b->npasses;
+  hipLaunchKernelGGL(sa::batch_clear, dim3((n + 255) / 256 ? (n + 255) / 256 : 1), dim3(256), 0, st,
+                     b->done, n, b->head);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return (int)e;
+  static bool attr[2] = {false, false};
+  if (!attr[prec ? 1 : 0]) {
+    const void* f = prec ? reinterpret_cast<const void*>(&sa::fft_batch_kernel<double>)
+                         : reinterpret_cast<const void*>(&sa::fft_batch_kernel<float>);
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr[prec ? 1 : 0] = true;
+  }
+  if (prec)
+    hipLaunchKernelGGL(sa::fft_batch_kernel<double>, dim3((uint32_t)grid), dim3((uint32_t)block), lds, st, *b);
+  else
+    hipLaunchKernelGGL(sa::fft_batch_kernel<float>, dim3((uint32_t)grid), dim3((uint32_t)block), lds, st, *b);
+  return (int)hipGetLastError();
+}
+
+extern "C" int sa_batch_occupancy(int prec, int block, size_t lds) {
+  const void* f = prec ? reinterpret_cast<const void*>(&sa::fft_batch_kernel<double>)
+                       : reinterpret_cast<const void*>(&sa::fft_batch_kernel<float>);
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, block, lds) != hipSuccess) return 0;
+  return blocks;
+}

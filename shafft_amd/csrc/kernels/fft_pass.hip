@@ -16,402 +16,14 @@
 //
 // Every pass reads and writes each element of its lines once, so the pass is
 // HBM-bound: algorithmic bytes per pass = 2 * batch * n * sizeof(T).
-#include "fft_device.hpp"
-#include "pass_desc.h"
+#include "pass_tile.hpp"
 
 namespace sa {
-
-// Register budget per thread per stage, in complex elements.
-constexpr int EMAX = 16;
-
-__device__ __forceinline__ int lp(int k) { return k + (k >> 4); }
-
-__device__ __forceinline__ uint32_t fdiv_d(uint32_t x, sa_fastdiv_t f) {
-  return fdiv(x, FastDiv{f.d, f.m, f.s});
-}
-
-template <typename R>
-struct Vec2;
-template <>
-struct Vec2<float> {
-  typedef float type __attribute__((ext_vector_type(2)));
-};
-template <>
-struct Vec2<double> {
-  typedef double type __attribute__((ext_vector_type(2)));
-};
-
-template <typename R>
-__device__ __forceinline__ cpx<R> gload(const cpx<R>* p, bool nt) {
-  typedef typename Vec2<R>::type V;
-  V v = nt ? __builtin_nontemporal_load(reinterpret_cast<const V*>(p)) : *reinterpret_cast<const V*>(p);
-  return {v.x, v.y};
-}
-
-template <typename R>
-__device__ __forceinline__ void gstore(cpx<R>* p, cpx<R> x, bool nt) {
-  typedef typename Vec2<R>::type V;
-  V v;
-  v.x = x.x;
-  v.y = x.y;
-  if (nt)
-    __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
-  else
-    *reinterpret_cast<V*>(p) = v;
-}
-
-// ---------------------------------------------------------------------------
-// Stockham stages.  Stage s (radix RAD, Ns = product of earlier radices):
-// butterfly jb reads k = jb + r*(n/RAD), twiddles by W_{Ns RAD}^{(jb%Ns) r},
-// and writes k = (jb/Ns)*Ns*RAD + jb%Ns + r*Ns.
-// ---------------------------------------------------------------------------
-
-// LDS -> LDS, butterflies jb = tl + i*T held in registers across a barrier.
-template <typename R, int RAD, int SIGN>
-__device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdiv_t nsd,
-                                          const cpx<R>* __restrict__ tw, int tl, int T) {
-  constexpr int MAXB = (EMAX + RAD - 1) / RAD;
-  const int nb = n / RAD;
-  const FastDiv fd{nsd.d, nsd.m, nsd.s};
-  cpx<R> v[MAXB][RAD];
-#pragma unroll
-  for (int i = 0; i < MAXB; ++i) {
-    const int jb = tl + i * T;
-    if (jb < nb) {
-#pragma unroll
-      for (int r = 0; r < RAD; ++r) v[i][r] = line[lp(jb + r * nb)];
-      if (Ns > 1) {
-        const int q = (int)fdiv((uint32_t)jb, fd);
-        const int rem = jb - q * Ns;
-        const cpx<R>* w = tw + rem * (RAD - 1);
-#pragma unroll
-        for (int r = 1; r < RAD; ++r) v[i][r] = cmul_dir<SIGN>(v[i][r], w[r - 1]);
-      }
-      dft<RAD, SIGN>(v[i]);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < MAXB; ++i) {
-    const int jb = tl + i * T;
-    if (jb < nb) {
-      const int q = (int)fdiv((uint32_t)jb, fd);
-      const int rem = jb - q * Ns;
-      const int idx = q * Ns * RAD + rem;
-#pragma unroll
-      for (int r = 0; r < RAD; ++r) line[lp(idx + r * Ns)] = v[i][r];
-    }
-  }
-  __syncthreads();
-}
-
-// Stage 0 from registers (v[m] = element tl + T*m, n == EMAX*T) into LDS.
-template <typename R, int RAD, int SIGN>
-__device__ __forceinline__ void stage_first_regs(const cpx<R> (&v)[EMAX], cpx<R>* line, int tl,
-                                                 int T) {
-  constexpr int MAXB = EMAX / RAD;
-#pragma unroll
-  for (int i = 0; i < MAXB; ++i) {
-    cpx<R> u[RAD];
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) u[r] = v[i + r * MAXB];
-    dft<RAD, SIGN>(u);
-    const int idx = (tl + i * T) * RAD;
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) line[lp(idx + r)] = u[r];
-  }
-  __syncthreads();
-}
-
-// Last stage (Ns = n/RAD) from LDS into registers: v[i + r*MAXB] receives
-// output element jb + r*Ns = tl + T*(i + r*MAXB).
-template <typename R, int RAD, int SIGN>
-__device__ __forceinline__ void stage_last_regs(cpx<R> (&v)[EMAX], const cpx<R>* line, int n,
-                                                const cpx<R>* __restrict__ tw, int tl, int T) {
-  constexpr int MAXB = EMAX / RAD;
-  const int nb = n / RAD;
-#pragma unroll
-  for (int i = 0; i < MAXB; ++i) {
-    const int jb = tl + i * T;
-    cpx<R> u[RAD];
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) u[r] = line[lp(jb + r * nb)];
-    const cpx<R>* w = tw + jb * (RAD - 1);
-#pragma unroll
-    for (int r = 1; r < RAD; ++r) u[r] = cmul_dir<SIGN>(u[r], w[r - 1]);
-    dft<RAD, SIGN>(u);
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) v[i + r * MAXB] = u[r];
-  }
-}
-
-template <typename R, int SIGN>
-__device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>* line, int tl) {
-  const cpx<R>* tws = reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[s];
-  switch (a.radix[s]) {
-    case 2: stage_lds<R, 2, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 3: stage_lds<R, 3, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 4: stage_lds<R, 4, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 5: stage_lds<R, 5, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 7: stage_lds<R, 7, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 8: stage_lds<R, 8, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 16: stage_lds<R, 16, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    default: break;
-  }
-}
-
-template <typename R, int SIGN>
-__device__ __forceinline__ void first_dispatch(const sa_pass_t& a, const cpx<R> (&v)[EMAX],
-                                               cpx<R>* line, int tl) {
-  switch (a.radix[0]) {
-    case 2: stage_first_regs<R, 2, SIGN>(v, line, tl, a.T); break;
-    case 4: stage_first_regs<R, 4, SIGN>(v, line, tl, a.T); break;
-    case 8: stage_first_regs<R, 8, SIGN>(v, line, tl, a.T); break;
-    case 16: stage_first_regs<R, 16, SIGN>(v, line, tl, a.T); break;
-    default: break;
-  }
-}
-
-template <typename R, int SIGN>
-__device__ __forceinline__ void last_dispatch(const sa_pass_t& a, cpx<R> (&v)[EMAX],
-                                              const cpx<R>* line, int tl) {
-  const int s = a.nstages - 1;
-  const cpx<R>* tws = reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[s];
-  switch (a.radix[s]) {
-    case 2: stage_last_regs<R, 2, SIGN>(v, line, a.n, tws, tl, a.T); break;
-    case 4: stage_last_regs<R, 4, SIGN>(v, line, a.n, tws, tl, a.T); break;
-    case 8: stage_last_regs<R, 8, SIGN>(v, line, a.n, tws, tl, a.T); break;
-    case 16: stage_last_regs<R, 16, SIGN>(v, line, a.n, tws, tl, a.T); break;
-    default: break;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Line addressing
-// ---------------------------------------------------------------------------
-struct LineOff {
-  int64_t in, out;
-  int64_t ld_base, st_base;  // digit terms of the load/store hooks
-  uint64_t ptw_dig;          // digit driving the four-step twiddle
-};
-
-// digit[lvl] as arithmetic (no indexed access, which would spill to scratch)
-__device__ __forceinline__ uint64_t sel3(int lvl, uint32_t d0, uint32_t d1, uint32_t d2) {
-  return (uint64_t)((lvl == 0) ? d0 : 0u) + (uint64_t)((lvl == 1) ? d1 : 0u) +
-         (uint64_t)((lvl == 2) ? d2 : 0u);
-}
-
-__device__ __forceinline__ LineOff line_offsets(const sa_pass_t& a, uint32_t b) {
-  LineOff o;
-  const uint32_t q2 = fdiv_d(b, a.n2div);
-  const uint32_t d2 = b - q2 * (uint32_t)a.n2;
-  const uint32_t d0 = fdiv_d(q2, a.n1div);
-  const uint32_t d1 = q2 - d0 * (uint32_t)a.n1;
-  o.in = (int64_t)d0 * a.is0 + (int64_t)d1 * a.is1 + (int64_t)d2 * a.is2;
-  o.out = (int64_t)d0 * a.os0 + (int64_t)d1 * a.os1 + (int64_t)d2 * a.os2;
-  o.ld_base = (int64_t)sel3(a.ld_level, d0, d1, d2) * a.ld_dmul +
-              (int64_t)sel3(a.ld_level2, d0, d1, d2) * a.ld_dmul2;
-  o.st_base = (int64_t)sel3(a.st_level, d0, d1, d2) * a.st_dmul +
-              (int64_t)sel3(a.st_level2, d0, d1, d2) * a.st_dmul2;
-  o.ptw_dig = sel3(a.ptw_level, d0, d1, d2);
-  return o;
-}
-
-// Line slot of thread t for a mapping kind: k-fast t = c*T + k0,
-// line-fast t = k0*C + c.
-__device__ __forceinline__ uint32_t slot_line(const sa_pass_t& a, uint32_t t, bool kfast) {
-  if (kfast) return fdiv_d(t, a.Tdiv);
-  return t - fdiv_d(t, a.Cdiv) * (uint32_t)a.C;
-}
-__device__ __forceinline__ uint32_t slot_elem(const sa_pass_t& a, uint32_t t, bool kfast) {
-  if (kfast) return t - fdiv_d(t, a.Tdiv) * (uint32_t)a.T;
-  return fdiv_d(t, a.Cdiv);
-}
-
-// Loads elements k = k0 + i*T (i < EMAX, k < n) of one line into v, with the
-// load hooks applied (zero outside the hook's limit).
-template <typename R>
-__device__ __forceinline__ void load_line(const sa_pass_t& a, const LineOff& o, uint32_t k0,
-                                          cpx<R> (&v)[EMAX]) {
-  const int n = a.n;
-  const uint32_t T = (uint32_t)a.T;
-  const cpx<R>* __restrict__ src = reinterpret_cast<const cpx<R>*>(a.in) + o.in;
-  const bool nt = a.nt != 0;
-  const bool plain = a.pre == nullptr && a.ld_level < 0 && a.ld_kmul == 1 && a.ld_limit >= n;
-  if (plain) {
-#pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
-      const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) v[i] = gload(src + (int64_t)k * a.iks, nt);
-    }
-  } else {
-    const cpx<R>* __restrict__ pre = reinterpret_cast<const cpx<R>*>(a.pre);
-    const int64_t base = o.ld_base;
-#pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
-      const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) {
-        const int64_t m = (int64_t)k * a.ld_kmul + base;
-        cpx<R> x = {R(0), R(0)};
-        if (m < a.ld_limit) {
-          x = gload(src + (int64_t)k * a.iks, nt);
-          if (pre) x = cmul(x, pre[m]);
-        }
-        v[i] = x;
-      }
-    }
-  }
-}
-
-// Stores elements k = k0 + i*T of one line from v with the store hooks.
-template <typename R, int SIGN>
-__device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o, uint32_t k0,
-                                           const cpx<R> (&v)[EMAX]) {
-  const int n = a.n;
-  const uint32_t T = (uint32_t)a.T;
-  cpx<R>* __restrict__ dst = reinterpret_cast<cpx<R>*>(a.out) + o.out;
-  const cpx<R>* __restrict__ post = reinterpret_cast<const cpx<R>*>(a.post);
-  const bool nt = a.nt != 0;
-  const bool do_scale = a.scale != 1.0;
-  const R sc = (R)a.scale;
-  if (a.ptw_level >= 0) {
-    // W_N^{(digit * k) mod N}, walked incrementally along k.
-    const uint64_t N = (uint64_t)a.ptw_N;
-    const uint64_t dig = o.ptw_dig;
-    uint64_t e = (dig * k0) % N;
-    const uint64_t step = (dig * T) % N;
-    const cpx<R>* __restrict__ lo = reinterpret_cast<const cpx<R>*>(a.ptw_lo);
-    const cpx<R>* __restrict__ hi = reinterpret_cast<const cpx<R>*>(a.ptw_hi);
-    const uint64_t mask = (1ull << a.ptw_shift) - 1;
-#pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
-      const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) {
-        const cpx<R> w = cmul(hi[e >> a.ptw_shift], lo[e & mask]);
-        cpx<R> x = cmul_dir<SIGN>(v[i], w);
-        if (do_scale) x = cscale(x, sc);
-        gstore(dst + (int64_t)k * a.oks, x, nt);
-      }
-      e += step;
-      if (e >= N) e -= N;
-    }
-  } else if (post == nullptr && a.st_level < 0 && a.st_kmul == 1 && a.st_limit >= n) {
-#pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
-      const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) {
-        cpx<R> x = v[i];
-        if (do_scale) x = cscale(x, sc);
-        gstore(dst + (int64_t)k * a.oks, x, nt);
-      }
-    }
-  } else {
-    const int64_t base = o.st_base;
-#pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
-      const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) {
-        const int64_t m = (int64_t)k * a.st_kmul + base;
-        if (m < a.st_limit) {
-          cpx<R> x = v[i];
-          if (post) x = cmul(x, post[m]);
-          if (do_scale) x = cscale(x, sc);
-          gstore(dst + (int64_t)k * a.oks, x, nt);
-        }
-      }
-    }
-  }
-}
-
-// Kernel variants: the generic LDS path, the register-resident first/last
-// stage path (both flags set, same lane mapping on both sides), Bluestein.
-enum PassMode { MODE_LDS = 0, MODE_REGS = 1, MODE_BLUE = 2 };
 
 template <typename R, int SIGN, int MODE>
 __global__ __launch_bounds__(512) void fft_lds_pass(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  cpx<R>* lds = reinterpret_cast<cpx<R>*>(lds_raw);
-  constexpr bool BLUE = MODE == MODE_BLUE;
-
-  const uint32_t t = threadIdx.x;
-  const uint32_t line0 = blockIdx.x * (uint32_t)a.C;
-  const int n = a.n;
-  constexpr bool reg_first = MODE == MODE_REGS;
-  constexpr bool reg_last = MODE == MODE_REGS;
-
-  // Thread identity for the transform: (line slot c, butterfly lane tl).
-  const bool id_kfast = reg_first ? (a.ld_kfast != 0) : (a.st_kfast != 0);
-  const uint32_t c = slot_line(a, t, id_kfast);
-  const uint32_t tl = slot_elem(a, t, id_kfast);
-  cpx<R>* line = lds + (size_t)c * a.ldsstride;
-  cpx<R> v[EMAX];
-
-  // ---- load ----------------------------------------------------------------
-  if constexpr (reg_first) {
-    const uint32_t b = line0 + c;
-    if (b < (uint32_t)a.batch) {
-      const LineOff o = line_offsets(a, b);
-      load_line<R>(a, o, tl, v);
-    } else {
-#pragma unroll
-      for (int i = 0; i < EMAX; ++i) v[i] = {R(0), R(0)};
-    }
-    first_dispatch<R, SIGN>(a, v, line, (int)tl);
-  } else {
-    const uint32_t lc = slot_line(a, t, a.ld_kfast != 0);
-    const uint32_t k0 = slot_elem(a, t, a.ld_kfast != 0);
-    const uint32_t b = line0 + lc;
-    if (b < (uint32_t)a.batch) {
-      const LineOff o = line_offsets(a, b);
-      cpx<R> u[EMAX];
-      load_line<R>(a, o, k0, u);
-      cpx<R>* ll = lds + (size_t)lc * a.ldsstride;
-#pragma unroll
-      for (int i = 0; i < EMAX; ++i) {
-        const uint32_t k = k0 + i * (uint32_t)a.T;
-        if (k < (uint32_t)n) ll[lp((int)k)] = u[i];
-      }
-    }
-    __syncthreads();
-  }
-
-  // ---- stages ----------------------------------------------------------------
-  if constexpr (!BLUE) {
-    const int s0 = reg_first ? 1 : 0;
-    const int s1 = reg_last ? a.nstages - 1 : a.nstages;
-    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN>(a, s, line, (int)tl);
-    if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl);
-  } else {
-    for (int s = 0; s < a.nstages; ++s) stage_dispatch<R, -1>(a, s, line, (int)tl);
-    const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
-    for (int k = (int)tl; k < n; k += a.T) line[lp(k)] = cmul(line[lp(k)], h[k]);
-    __syncthreads();
-    for (int s = 0; s < a.nstages; ++s) stage_dispatch<R, +1>(a, s, line, (int)tl);
-  }
-
-  // ---- store -----------------------------------------------------------------
-  if constexpr (reg_last) {
-    const uint32_t b = line0 + c;
-    if (b < (uint32_t)a.batch) {
-      const LineOff o = line_offsets(a, b);
-      store_line<R, SIGN>(a, o, tl, v);
-    }
-  } else {
-    const uint32_t sc = slot_line(a, t, a.st_kfast != 0);
-    const uint32_t k0 = slot_elem(a, t, a.st_kfast != 0);
-    const uint32_t b = line0 + sc;
-    if (b < (uint32_t)a.batch) {
-      const cpx<R>* ll = lds + (size_t)sc * a.ldsstride;
-      cpx<R> u[EMAX];
-#pragma unroll
-      for (int i = 0; i < EMAX; ++i) {
-        const uint32_t k = k0 + i * (uint32_t)a.T;
-        if (k < (uint32_t)n) u[i] = ll[lp((int)k)];
-      }
-      const LineOff o = line_offsets(a, b);
-      store_line<R, SIGN>(a, o, k0, u);
-    }
-  }
+  pass_tile<R, SIGN, MODE>(a, blockIdx.x, reinterpret_cast<cpx<R>*>(lds_raw));
 }
 
 template <typename R>

@@ -99,6 +99,35 @@ int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid, void* stre
 /* Resident workgroups per CU of that kernel for a block size and LDS size. */
 int sa_fixed_occupancy(int n, int prec, int sign, int nthr, size_t lds);
 
+/* Batched persistent launch: many passes of many transforms in one grid.
+ * Workgroups pull items (pass, tile) from a device queue in list order;
+ * an item waits until every tile of the pass it depends on has completed.
+ * The list must be ordered so that every item's dependency pass has all of
+ * its tiles earlier in the list (level-major order does this).          */
+typedef struct {
+  uint32_t pass;
+  uint32_t tile;
+} sa_batch_item_t;
+
+typedef struct {
+  const sa_pass_t* passes;      /* pass descriptors (device)                  */
+  const uint32_t* pass_tiles;   /* tiles per pass (device)                    */
+  const int32_t* pass_dep;      /* pass each pass waits for, -1 none (device) */
+  const int8_t* pass_sign;      /* -1 / +1 per pass (device)                  */
+  uint32_t* done;               /* completed tiles per pass, zero at launch   */
+  uint32_t* head;               /* queue head, zero at launch                 */
+  const sa_batch_item_t* items; /* work list (device)                         */
+  uint32_t nitems;
+  uint32_t npasses;
+} sa_batch_t;
+
+/* Zero the queue head and the per-pass counters (a kernel, so it can sit in
+ * a captured graph), then run the persistent batch kernel with `grid`
+ * workgroups of `block` threads and `lds` bytes of dynamic LDS. */
+int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int block, size_t lds, void* stream);
+/* Resident batch workgroups per CU for a block size and LDS size. */
+int sa_batch_occupancy(int prec, int block, size_t lds);
+
 /* x[i] *= f for i < count complex elements. */
 int sa_launch_scale(void* data, double f, uint64_t count, int prec, void* stream);
 

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <array>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <vector>
 #include <cstring>
@@ -264,6 +265,50 @@ extern "C" int sa_launch_scale(void* data, double f, uint64_t count, int prec, v
   return 0;
 }
 
+// ---- batch launch: run the work list in order, one whole pass at the first
+// item of each pass, after checking the list's dependency order ---------------
+namespace {
+bool g_capturing = false;
+std::vector<std::function<void()>>* g_capture = nullptr;
+}
+
+extern "C" int sa_batch_occupancy(int, int, size_t) { return 1; }
+
+extern "C" int sa_launch_batch(const sa_batch_t* bp, int prec, int grid, int block, size_t lds,
+                               void*) {
+  const sa_batch_t b = *bp;
+  auto run = [b, prec, grid, block, lds]() {
+    auto die = [](const char* m) {
+      std::fprintf(stderr, "EMU BAD BATCH: %s\n", m);
+      std::abort();
+    };
+    if (grid < 1 || block < 1 || block > 512 || lds > 160 * 1024) die("launch shape");
+    std::vector<uint32_t> seen(b.npasses, 0);
+    std::vector<char> ran(b.npasses, 0);
+    for (uint32_t it = 0; it < b.nitems; ++it) {
+      const sa_batch_item_t w = b.items[it];
+      if (w.pass >= b.npasses || w.tile >= b.pass_tiles[w.pass]) die("item out of range");
+      const int dep = b.pass_dep[w.pass];
+      if (dep >= 0 && seen[dep] != b.pass_tiles[dep]) die("item precedes its dependency");
+      const sa_pass_t& p = b.passes[w.pass];
+      if (p.C * p.T > block) die("pass tile larger than the workgroup");
+      if ((size_t)p.C * p.ldsstride * (prec ? 16 : 8) > lds) die("pass LDS larger than the launch");
+      if (!ran[w.pass]) {
+        sa_launch_pass(&p, prec, b.pass_sign[w.pass], nullptr);
+        ran[w.pass] = 1;
+      }
+      ++seen[w.pass];
+    }
+    for (uint32_t i = 0; i < b.npasses; ++i)
+      if (seen[i] != b.pass_tiles[i]) die("pass not fully covered by the work list");
+  };
+  if (g_capturing)
+    g_capture->push_back(run);
+  else
+    run();
+  return 0;
+}
+
 // ---- the HIP runtime calls used by the host code, on host memory ----------
 extern "C" {
 hipError_t hipMalloc(void** p, size_t n) {
@@ -281,9 +326,53 @@ hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
   return hipSuccess;
 }
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+  if (g_capturing) {
+    g_capture->push_back([d, s, n]() { std::memmove(d, s, n); });
+    return hipSuccess;
+  }
   std::memmove(d, s, n);
   return hipSuccess;
 }
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) {
+  *s = reinterpret_cast<hipStream_t>(0x1);
+  return hipSuccess;
+}
+hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  g_capturing = true;
+  g_capture = new std::vector<std::function<void()>>();
+  return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+  g_capturing = false;
+  *g = reinterpret_cast<hipGraph_t>(g_capture);
+  g_capture = nullptr;
+  return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
+  *e = reinterpret_cast<hipGraphExec_t>(g);
+  return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto& f : *reinterpret_cast<std::vector<std::function<void()>>*>(e)) f();
+  return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+hipError_t hipGraphDestroy(hipGraph_t g) {
+  delete reinterpret_cast<std::vector<std::function<void()>>*>(g);
+  return hipSuccess;
+}
+hipError_t hipEventCreate(hipEvent_t* e) {
+  *e = reinterpret_cast<hipEvent_t>(0x1);
+  return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
+  *ms = 1.0f;
+  return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "emulated"; }
 hipError_t hipGetDevice(int* d) {
