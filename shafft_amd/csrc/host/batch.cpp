@@ -166,7 +166,7 @@ void BatchPlan::build() {
   up(d_dep_, dep.data(), np * sizeof(int32_t));
   up(d_items_, items_.data(), items_.size() * sizeof(sa_batch_item_t));
   if (d_done_.bytes < std::max<size_t>(np, 1) * 4) d_done_ = DevBuf(std::max<size_t>(np, 1) * 4);
-  if (!d_head_.p) d_head_ = DevBuf(256);
+  if (!d_head_.p) d_head_ = DevBuf(256);  // [0] queue head, [1] wait-timeout flag
   built_ = true;
 }
 
@@ -192,6 +192,10 @@ void BatchPlan::launch(int dir, hipStream_t s) {
   b.pass_sign = static_cast<const int8_t*>(d_sign_[d].p);
   b.done = static_cast<uint32_t*>(d_done_.p);
   b.head = static_cast<uint32_t*>(d_head_.p);
+  b.err = static_cast<uint32_t*>(d_head_.p) + 1;
+  // ~2^26 polls of s_sleep 2 (>= 128 clocks each): seconds, far beyond any
+  // legitimate wait for one pass of one transform.
+  b.spin_limit = 1u << 26;
   b.items = static_cast<const sa_batch_item_t*>(d_items_.p);
   b.nitems = (uint32_t)items_.size();
   b.npasses = (uint32_t)prog_.npasses();
@@ -223,6 +227,15 @@ void BatchPlan::execute(int dir, bool use_graph) {
     hip_check(hipGraphInstantiate(&graph_[d], gdef_[d], nullptr, nullptr, 0), "hipGraphInstantiate");
   }
   hip_check(hipGraphLaunch(graph_[d], stream_), "hipGraphLaunch");
+}
+
+void BatchPlan::synchronize() {
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  if (!d_head_.p) return;
+  uint32_t err = 0;
+  hip_check(hipMemcpy(&err, static_cast<uint32_t*>(d_head_.p) + 1, 4, hipMemcpyDeviceToHost),
+            "hipMemcpy(batch flag)");
+  if (err) fail(ERR_BACKEND, "batch launch: a dependency wait timed out");
 }
 
 int64_t BatchPlan::total_bytes() const { return prog_.total_bytes(); }

@@ -39,6 +39,8 @@ class BatchPlan {
   hipStream_t stream() const { return stream_; }
   // dir -1 forward / +1 backward; graph replay unless use_graph is false.
   void execute(int dir, bool use_graph);
+  // Wait for the stream; fails if the last launch reported a timed-out wait.
+  void synchronize();
   // Time candidate schedules on the attached buffers and keep the fastest.
   std::string autotune();
   const BatchSchedule& schedule() const { return sched_; }

@@ -337,7 +337,7 @@ int shafftAmdBatchExecute(shafft_amd_batch_plan* plan, int direction, int use_gr
 int shafftAmdBatchSynchronize(shafft_amd_batch_plan* plan) {
   return guarded([&] {
     if (!plan) fail(ERR_NULLPTR, "null plan");
-    hip_check(hipStreamSynchronize(plan->be->stream()), "hipStreamSynchronize");
+    plan->be->synchronize();
     return 0;
   });
 }

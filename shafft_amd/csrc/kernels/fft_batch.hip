@@ -16,10 +16,13 @@
 
 namespace sa {
 
-__global__ void batch_clear(uint32_t* done, uint32_t n, uint32_t* head) {
+__global__ void batch_clear(uint32_t* done, uint32_t n, uint32_t* head, uint32_t* err) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) done[i] = 0u;
-  if (i == 0) *head = 0u;
+  if (i == 0) {
+    *head = 0u;
+    *err = 0u;
+  }
 }
 
 template <typename R>
@@ -38,9 +41,17 @@ __global__ __launch_bounds__(512) void fft_batch_kernel(const sa_batch_t b) {
     const int dep = b.pass_dep[w.pass];
     if (dep >= 0) {
       if (threadIdx.x == 0) {
+        // Bounded wait: a wait that outlives spin_limit polls flags the
+        // launch as failed instead of holding the GPU.
         const uint32_t need = b.pass_tiles[dep];
-        while (__hip_atomic_load(&b.done[dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
-          __builtin_amdgcn_s_sleep(1);
+        uint32_t polls = 0;
+        while (__hip_atomic_load(&b.done[dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++polls > b.spin_limit) {
+            __hip_atomic_store(b.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
       }
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -66,7 +77,7 @@ extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int bloc
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const uint32_t n = b->npasses;
   hipLaunchKernelGGL(sa::batch_clear, dim3((n + 255) / 256 ? (n + 255) / 256 : 1), dim3(256), 0, st,
-                     b->done, n, b->head);
+                     b->done, n, b->head, b->err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return (int)e;
   static bool attr[2] = {false, false};
