@@ -20,6 +20,18 @@ static int cu_count() {
   return cached;
 }
 
+static uint64_t wall_clock_hz() {
+  static uint64_t cached = 0;
+  if (!cached) {
+    int dev = 0, khz = 0;
+    cached = 100000000ull;  // gfx9 wall clock: 100 MHz
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+      cached = (uint64_t)khz * 1000ull;
+  }
+  return cached;
+}
+
 BatchPlan::BatchPlan(int count, const int* ndims, const int* dims_flat, int prec)
     : prec_(prec), prog_(prec) {
   if (prec != 0 && prec != 1) fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
@@ -93,7 +105,7 @@ void BatchPlan::build() {
   sign[1].resize(np);
   block_ = 1;
   lds_ = 0;
-  const int budget = 160 * 1024;
+  const int budget = 159 * 1024;
   for (const Tr& tr : tr_) {
     for (int j = 0; j < tr.np; ++j) {
       const size_t i = (size_t)(tr.first + j);
@@ -193,9 +205,9 @@ void BatchPlan::launch(int dir, hipStream_t s) {
   b.done = static_cast<uint32_t*>(d_done_.p);
   b.head = static_cast<uint32_t*>(d_head_.p);
   b.err = static_cast<uint32_t*>(d_head_.p) + 1;
-  // ~2^26 polls of s_sleep 2 (>= 128 clocks each): seconds, far beyond any
-  // legitimate wait for one pass of one transform.
-  b.spin_limit = 1u << 26;
+  // Two seconds of the device wall clock: far beyond any legitimate wait
+  // for one pass of one transform.
+  b.wait_ticks = 2ull * wall_clock_hz();
   b.items = static_cast<const sa_batch_item_t*>(d_items_.p);
   b.nitems = (uint32_t)items_.size();
   b.npasses = (uint32_t)prog_.npasses();

@@ -30,24 +30,27 @@ __global__ __launch_bounds__(512) void fft_batch_kernel(const sa_batch_t b) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ uint32_t s_item;
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(lds_raw);
+  // Every branch and loop bound below that encloses a barrier is made
+  // wave-uniform with readfirstlane (see uni() in pass_tile.hpp).
   for (;;) {
     if (threadIdx.x == 0)
       s_item = __hip_atomic_fetch_add(b.head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const uint32_t it = s_item;
+    const uint32_t it = (uint32_t)uni((int)s_item);
     __syncthreads();
     if (it >= b.nitems) break;
-    const sa_batch_item_t w = b.items[it];
-    const int dep = b.pass_dep[w.pass];
+    const uint32_t pass = (uint32_t)uni((int)b.items[it].pass);
+    const uint32_t tile = (uint32_t)uni((int)b.items[it].tile);
+    const int dep = uni(b.pass_dep[pass]);
     if (dep >= 0) {
       if (threadIdx.x == 0) {
-        // Bounded wait: a wait that outlives spin_limit polls flags the
-        // launch as failed instead of holding the GPU.
+        // Bounded wait (wall clock): a wait that outlives wait_ticks flags
+        // the launch as failed instead of holding the GPU.
         const uint32_t need = b.pass_tiles[dep];
-        uint32_t polls = 0;
+        const uint64_t t0 = wall_clock64();
         while (__hip_atomic_load(&b.done[dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
           __builtin_amdgcn_s_sleep(2);
-          if (++polls > b.spin_limit) {
+          if (wall_clock64() - t0 > b.wait_ticks) {
             __hip_atomic_store(b.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
           }
@@ -56,17 +59,19 @@ __global__ __launch_bounds__(512) void fft_batch_kernel(const sa_batch_t b) {
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    const sa_pass_t& a = b.passes[w.pass];
-    if (a.blu_h)
-      pass_tile<R, -1, MODE_BLUE>(a, w.tile, lds);
-    else if (b.pass_sign[w.pass] < 0)
-      pass_tile<R, -1, MODE_LDS>(a, w.tile, lds);
+    const sa_pass_t& a = b.passes[pass];
+    const int kind = uni(a.blu_h != nullptr ? 2 : (b.pass_sign[pass] < 0 ? 0 : 1));
+    if (kind == 2)
+      pass_tile<R, -1, MODE_BLUE>(a, tile, lds);
+    else if (kind == 0)
+      pass_tile<R, -1, MODE_LDS>(a, tile, lds);
     else
-      pass_tile<R, +1, MODE_LDS>(a, w.tile, lds);
+      pass_tile<R, +1, MODE_LDS>(a, tile, lds);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0)
-      __hip_atomic_fetch_add(&b.done[w.pass], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&b.done[pass], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
   }
 }
 
@@ -84,7 +89,9 @@ extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int bloc
   if (!attr[prec ? 1 : 0]) {
     const void* f = prec ? reinterpret_cast<const void*>(&sa::fft_batch_kernel<double>)
                          : reinterpret_cast<const void*>(&sa::fft_batch_kernel<float>);
-    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    // 160 KiB minus room for the kernel's static LDS (the queue item slot).
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
+    (void)hipGetLastError();
     attr[prec ? 1 : 0] = true;
   }
   if (prec)
@@ -97,7 +104,8 @@ extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int bloc
 extern "C" int sa_batch_occupancy(int prec, int block, size_t lds) {
   const void* f = prec ? reinterpret_cast<const void*>(&sa::fft_batch_kernel<double>)
                        : reinterpret_cast<const void*>(&sa::fft_batch_kernel<float>);
-  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
+  (void)hipGetLastError();
   int blocks = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, block, lds) != hipSuccess) return 0;
   return blocks;

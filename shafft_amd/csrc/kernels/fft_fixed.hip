@@ -349,6 +349,7 @@ hipError_t launch_n(const sa_pass_t* p, hipStream_t s, int grid) {
   if (!attr) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_fixed_pass<R, N, SIGN>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipGetLastError();
     attr = true;
   }
   hipLaunchKernelGGL((fft_fixed_pass<R, N, SIGN>), dim3((uint32_t)grid), dim3(nthr), lds, s, *p);
@@ -418,6 +419,7 @@ extern "C" int sa_fixed_occupancy(int n, int prec, int sign, int nthr, size_t ld
   }
 #undef SA_FX_CASE
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipGetLastError();
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, nthr, lds) != hipSuccess) return 0;
   return blocks;
 }

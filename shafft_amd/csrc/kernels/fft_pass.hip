@@ -46,6 +46,7 @@ static hipError_t launch_t(const sa_pass_t* p, hipStream_t st) {
   if (!attr_done) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_pass<R, SIGN, MODE>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, sa_max_lds_bytes());
+    (void)hipGetLastError();
     attr_done = true;
   }
   hipLaunchKernelGGL((fft_lds_pass<R, SIGN, MODE>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);

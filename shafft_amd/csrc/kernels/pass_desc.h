@@ -117,7 +117,7 @@ typedef struct {
   uint32_t* done;               /* completed tiles per pass, zero at launch   */
   uint32_t* head;               /* queue head, zero at launch                 */
   uint32_t* err;                /* set to 1 if a dependency wait timed out    */
-  uint32_t spin_limit;          /* polls before a dependency wait gives up    */
+  uint64_t wait_ticks;          /* wall-clock ticks before a wait gives up    */
   const sa_batch_item_t* items; /* work list (device)                         */
   uint32_t nitems;
   uint32_t npasses;

@@ -13,6 +13,12 @@ constexpr int EMAX = 16;
 
 __device__ __forceinline__ int lp(int k) { return k + (k >> 4); }
 
+// Wave-uniform copy of a value that is the same in every lane (descriptor
+// fields read from memory).  Control flow that encloses a barrier must be
+// provably uniform, or the compiler may give lanes of one wave different
+// loop exits and the wave reaches the barriers out of step.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ uint32_t fdiv_d(uint32_t x, sa_fastdiv_t f) {
   return fdiv(x, FastDiv{f.d, f.m, f.s});
 }
@@ -136,7 +142,7 @@ __device__ __forceinline__ void stage_last_regs(cpx<R> (&v)[EMAX], const cpx<R>*
 template <typename R, int SIGN>
 __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>* line, int tl) {
   const cpx<R>* tws = reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[s];
-  switch (a.radix[s]) {
+  switch (uni(a.radix[s])) {
     case 2: stage_lds<R, 2, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
     case 3: stage_lds<R, 3, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
     case 4: stage_lds<R, 4, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
@@ -151,7 +157,7 @@ __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>
 template <typename R, int SIGN>
 __device__ __forceinline__ void first_dispatch(const sa_pass_t& a, const cpx<R> (&v)[EMAX],
                                                cpx<R>* line, int tl, bool act) {
-  switch (a.radix[0]) {
+  switch (uni(a.radix[0])) {
     case 2: stage_first_regs<R, 2, SIGN>(v, line, tl, a.T, act); break;
     case 4: stage_first_regs<R, 4, SIGN>(v, line, tl, a.T, act); break;
     case 8: stage_first_regs<R, 8, SIGN>(v, line, tl, a.T, act); break;
@@ -370,15 +376,16 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   // ---- stages ----------------------------------------------------------------
   if constexpr (!BLUE) {
     const int s0 = reg_first ? 1 : 0;
-    const int s1 = reg_last ? a.nstages - 1 : a.nstages;
+    const int s1 = reg_last ? uni(a.nstages) - 1 : uni(a.nstages);
     for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN>(a, s, line, (int)tl);
     if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl, active);
   } else {
-    for (int s = 0; s < a.nstages; ++s) stage_dispatch<R, -1>(a, s, line, (int)tl);
+    const int nst = uni(a.nstages);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, -1>(a, s, line, (int)tl);
     const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
     for (int k = (int)tl; k < n; k += a.T) line[lp(k)] = cmul(line[lp(k)], h[k]);
     __syncthreads();
-    for (int s = 0; s < a.nstages; ++s) stage_dispatch<R, +1>(a, s, line, (int)tl);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, +1>(a, s, line, (int)tl);
   }
 
   // ---- store -----------------------------------------------------------------

@@ -282,8 +282,8 @@ extern "C" int sa_launch_batch(const sa_batch_t* bp, int prec, int grid, int blo
       std::fprintf(stderr, "EMU BAD BATCH: %s\n", m);
       std::abort();
     };
-    if (grid < 1 || block < 1 || block > 512 || lds > 160 * 1024) die("launch shape");
-    if (!b.err || b.spin_limit == 0) die("missing wait bound");
+    if (grid < 1 || block < 1 || block > 512 || lds > 159 * 1024) die("launch shape");
+    if (!b.err || b.wait_ticks == 0) die("missing wait bound");
     *b.err = 0;
     std::vector<uint32_t> seen(b.npasses, 0);
     std::vector<char> ran(b.npasses, 0);
