@@ -106,6 +106,91 @@ def profiled_traffic(dims, prec):
     return None
 
 
+def bench_batch(args, hip):
+    """Batched uneven workload: `count` complex64 2-D/3-D transforms, axes
+    log-uniform over the 7-smooth lengths in [16, 256].  Reports the batched
+    graph launch against the plain per-call path (one FFTND plan per shape,
+    host-synchronous execute per transform, as the reference API runs)."""
+    from shafft_amd import api
+    from tests.batch_shapes import draw_shapes
+
+    shapes = draw_shapes(args.count, 45)
+    prec = api.C2C
+    bp = api.BatchPlan(shapes, prec)
+    tot = bp.total
+    rng = np.random.default_rng(45)
+    x = np.empty(tot, np.complex64)
+    x.real = rng.uniform(-1, 1, tot)
+    x.imag = rng.uniform(-1, 1, tot)
+    a, o, w = (api.DeviceBuffer(tot, prec) for _ in range(3))
+    a.upload(x)
+    stream = hip.stream()
+    bp.set_stream(stream)
+    bp.set_buffers(a.ptr, o.ptr, w.ptr)
+    tune = bp.autotune() if not args.no_autotune else "defaults"
+    flops = sum(5 * int(np.prod(sh)) * math.log2(int(np.prod(sh))) for sh in shapes)
+    for _ in range(args.warmup):
+        bp.execute(api.FORWARD)
+    bp.synchronize()
+    e0, e1 = hip.event(), hip.event()
+    t0 = time.perf_counter()
+    hip.record(e0, stream)
+    for _ in range(args.steps):
+        bp.execute(api.FORWARD)
+    hip.record(e1, stream)
+    bp.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    ms = hip.elapsed_ms(e0, e1) / args.steps
+    info = bp.info()
+    # plain per-call path
+    plans = []
+    for i, sh in enumerate(shapes):
+        p = api.FFTND()
+        p.init([1] * len(sh), sh, prec)
+        p.plan()
+        p.set_stream(stream)
+        off = bp.offsets[i] * 8
+        p.set_buffers(a.ptr + off, o.ptr + off)
+        plans.append(p)
+    for p in plans[:64]:
+        p.execute(api.FORWARD)
+    hip.sync()
+    t0 = time.perf_counter()
+    for p in plans:
+        p.execute(api.FORWARD)   # host-synchronous, result swapped into data
+    hip.sync()
+    plain = time.perf_counter() - t0
+    out = {
+        "metric": "batched uneven FFTND throughput (GFLOP/s, 5 N log2 N per transform)",
+        "value": round(flops / (wall * 1e-3 * 1e3) / 1e9, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(shapes)} complex64 2-D/3-D transforms, axes log-uniform over the 64 7-smooth "
+                        f"lengths in [16,256], one persistent graph launch per step (forward)",
+            "elements": tot,
+            "gpu_ms_per_batch": round(ms, 4),
+            "plain_per_call_ms": round(plain * 1e3, 2),
+            "speedup_vs_plain": round(plain / wall, 2),
+            "plain_gflops": round(flops / plain / 1e9, 2),
+            "hbm_gbs": round(bp.info()["bytes"] / (ms * 1e-3) / 1e9, 1),
+            "passes": info["passes"],
+            "items": info["items"],
+            "grid": info["grid"],
+            "schedule": tune.strip().splitlines()[-1] if tune else "",
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +199,9 @@ def main():
     ap.add_argument("--dims", type=str, default="512,512,512")
     ap.add_argument("--prec", type=str, default="z2z", choices=["c2c", "z2z"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="fftnd", choices=["fftnd", "batch"])
+    ap.add_argument("--count", type=int, default=4096, help="transforms in the batch workload")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one device-to-device copy of the data buffer first (known-size "
                          "reference kernel for HBM counter scaling under rocprofv3 --pmc)")
@@ -122,6 +210,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU per process; multi-GPU runs "
                                    "are not part of it (see DESIGN.md, sharding)"}))
         sys.exit(2)
+
+    if args.workload == "batch":
+        bench_batch(args, Hip())
+        return
 
     from shafft_amd import api
 

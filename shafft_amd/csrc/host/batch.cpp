@@ -153,14 +153,18 @@ void BatchPlan::build() {
   }
   // Level-major work list.
   items_.clear();
+  level_start_.clear();
   int maxnp = 0;
   for (const Tr& tr : tr_) maxnp = std::max(maxnp, tr.np);
-  for (int lvl = 0; lvl < maxnp; ++lvl)
+  for (int lvl = 0; lvl < maxnp; ++lvl) {
+    level_start_.push_back(items_.size());
     for (const Tr& tr : tr_)
       if (lvl < tr.np) {
         const uint32_t pi = (uint32_t)(tr.first + lvl);
         for (uint32_t k = 0; k < tiles[pi]; ++k) items_.push_back({pi, k});
       }
+  }
+  level_start_.push_back(items_.size());
   const int occ = std::max(1, sa_batch_occupancy(prec_, block_, lds_));
   grid_ = (int)std::min<int64_t>((int64_t)items_.size(),
                                  (int64_t)sched_.oversub * occ * cu_count());
@@ -208,10 +212,25 @@ void BatchPlan::launch(int dir, hipStream_t s) {
   // Two seconds of the device wall clock: far beyond any legitimate wait
   // for one pass of one transform.
   b.wait_ticks = 2ull * wall_clock_hz();
-  b.items = static_cast<const sa_batch_item_t*>(d_items_.p);
-  b.nitems = (uint32_t)items_.size();
   b.npasses = (uint32_t)prog_.npasses();
-  hip_check((hipError_t)sa_launch_batch(&b, prec_, grid_, block_, lds_, s), "batch launch");
+  const sa_batch_item_t* all = static_cast<const sa_batch_item_t*>(d_items_.p);
+  if (!sched_.levels) {
+    b.items = all;
+    b.nitems = (uint32_t)items_.size();
+    b.independent = 0;
+    hip_check((hipError_t)sa_launch_batch(&b, prec_, grid_, block_, lds_, s), "batch launch");
+    return;
+  }
+  // Level mode: one persistent launch per pass level, in stream order.
+  b.independent = 1;
+  for (size_t l = 0; l + 1 < level_start_.size(); ++l) {
+    const size_t n = level_start_[l + 1] - level_start_[l];
+    if (!n) continue;
+    b.items = all + level_start_[l];
+    b.nitems = (uint32_t)n;
+    const int g = (int)std::min<int64_t>((int64_t)n, grid_);
+    hip_check((hipError_t)sa_launch_batch(&b, prec_, g, block_, lds_, s), "batch launch");
+  }
 }
 
 void BatchPlan::execute(int dir, bool use_graph) {
@@ -254,7 +273,8 @@ int64_t BatchPlan::total_bytes() const { return prog_.total_bytes(); }
 
 std::string BatchPlan::autotune() {
   if (!built_) fail(ERR_NO_BUFFER, "batch buffers not set");
-  const BatchSchedule cands[] = {{256, 1}, {256, 2}, {512, 1}, {128, 2}};
+  const BatchSchedule cands[] = {{512, 1, true},  {256, 1, true}, {256, 2, true},
+                                 {512, 1, false}, {256, 1, false}};
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");
@@ -273,8 +293,8 @@ std::string BatchPlan::autotune() {
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
     ms /= 3;
-    os << "block=" << c.block << " oversub=" << c.oversub << " grid=" << grid_ << " : " << ms
-       << " ms\n";
+    os << "block=" << c.block << " oversub=" << c.oversub << " mode=" << (c.levels ? "levels" : "deps")
+       << " grid=" << grid_ << " : " << ms << " ms\n";
     if (ms < best) {
       best = ms;
       best_s = c;
@@ -285,7 +305,8 @@ std::string BatchPlan::autotune() {
   sched_ = best_s;
   drop_graphs();
   build();
-  os << "chosen block=" << sched_.block << " oversub=" << sched_.oversub << "\n";
+  os << "chosen block=" << sched_.block << " oversub=" << sched_.oversub
+     << " mode=" << (sched_.levels ? "levels" : "deps") << "\n";
   return os.str();
 }
 

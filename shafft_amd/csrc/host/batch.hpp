@@ -20,8 +20,9 @@
 namespace sa {
 
 struct BatchSchedule {
-  int block = 256;   // threads per workgroup (tile = block / T lines)
-  int oversub = 1;   // persistent workgroups = oversub * resident capacity
+  int block = 512;    // threads per workgroup (tile = block / T lines)
+  int oversub = 1;    // persistent workgroups = oversub * resident capacity
+  bool levels = true; // one launch per pass level instead of dependency waits
 };
 
 class BatchPlan {
@@ -64,6 +65,7 @@ class BatchPlan {
   int grid_ = 0, block_ = 0;
   size_t lds_ = 0;
   std::vector<sa_batch_item_t> items_;
+  std::vector<size_t> level_start_;  // items_ index where each level begins (+ end)
   DevBuf d_pass_[2], d_tiles_, d_dep_, d_sign_[2], d_done_, d_head_, d_items_;
   hipGraphExec_t graph_[2] = {nullptr, nullptr};
   hipGraph_t gdef_[2] = {nullptr, nullptr};

@@ -1,5 +1,6 @@
 // Persistent batched FFT kernel for gfx950: many transforms of uneven shape
-// in one launch.
+// in one launch (dependency mode) or one launch per pass level (level mode,
+// all launches in one captured graph).
 //
 // The host flattens every pass of every transform into a work list of
 // (pass, tile) items in level-major order (all first passes, then all second
@@ -9,7 +10,10 @@
 // until that pass's completed-tile counter is full; a finished tile makes
 // its stores visible with an agent-scope release before counting itself.
 // Every dependency of an item was handed out before the item itself, to a
-// workgroup that is already running, so the queue always drains.
+// workgroup that is already running, so the queue always drains.  The
+// agent-scope release is an L2 write-back on gfx950, so level mode, where
+// the kernel boundary orders the levels, is usually faster; the autotuner
+// times both.
 #include "fft_device.hpp"
 #include "pass_desc.h"
 #include "pass_tile.hpp"
@@ -41,7 +45,7 @@ __global__ __launch_bounds__(512) void fft_batch_kernel(const sa_batch_t b) {
     if (it >= b.nitems) break;
     const uint32_t pass = (uint32_t)uni((int)b.items[it].pass);
     const uint32_t tile = (uint32_t)uni((int)b.items[it].tile);
-    const int dep = uni(b.pass_dep[pass]);
+    const int dep = b.independent ? -1 : uni(b.pass_dep[pass]);
     if (dep >= 0) {
       if (threadIdx.x == 0) {
         // Bounded wait (wall clock): a wait that outlives wait_ticks flags
@@ -67,10 +71,12 @@ __global__ __launch_bounds__(512) void fft_batch_kernel(const sa_batch_t b) {
       pass_tile<R, -1, MODE_LDS>(a, tile, lds);
     else
       pass_tile<R, +1, MODE_LDS>(a, tile, lds);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0)
-      __hip_atomic_fetch_add(&b.done[pass], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (!b.independent) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(&b.done[pass], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
     __syncthreads();
   }
 }

@@ -121,6 +121,9 @@ typedef struct {
   const sa_batch_item_t* items; /* work list (device)                         */
   uint32_t nitems;
   uint32_t npasses;
+  /* 1: the list holds one dependency level only (the launch boundary is the
+   * synchronisation): no waits and no per-tile release.                   */
+  uint32_t independent;
 } sa_batch_t;
 
 /* Zero the queue head and the per-pass counters (a kernel, so it can sit in

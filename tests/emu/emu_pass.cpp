@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <functional>
 #include <map>
+#include <set>
 #include <vector>
 #include <cstring>
 #include <algorithm>
@@ -268,6 +269,7 @@ extern "C" int sa_launch_scale(void* data, double f, uint64_t count, int prec, v
 // ---- batch launch: run the work list in order, one whole pass at the first
 // item of each pass, after checking the list's dependency order ---------------
 namespace {
+std::set<int> g_done_pass;  // passes completed by earlier level launches
 bool g_capturing = false;
 std::vector<std::function<void()>>* g_capture = nullptr;
 }
@@ -285,13 +287,19 @@ extern "C" int sa_launch_batch(const sa_batch_t* bp, int prec, int grid, int blo
     if (grid < 1 || block < 1 || block > 512 || lds > 159 * 1024) die("launch shape");
     if (!b.err || b.wait_ticks == 0) die("missing wait bound");
     *b.err = 0;
+    bool first_level = true;
+    for (uint32_t it = 0; it < b.nitems; ++it)
+      if (b.pass_dep[b.items[it].pass] >= 0) first_level = false;
+    if (!b.independent || first_level) g_done_pass.clear();
     std::vector<uint32_t> seen(b.npasses, 0);
     std::vector<char> ran(b.npasses, 0);
     for (uint32_t it = 0; it < b.nitems; ++it) {
       const sa_batch_item_t w = b.items[it];
       if (w.pass >= b.npasses || w.tile >= b.pass_tiles[w.pass]) die("item out of range");
       const int dep = b.pass_dep[w.pass];
-      if (dep >= 0 && seen[dep] != b.pass_tiles[dep]) die("item precedes its dependency");
+      if (!b.independent && dep >= 0 && seen[dep] != b.pass_tiles[dep])
+        die("item precedes its dependency");
+      if (b.independent && dep >= 0 && !g_done_pass.count(dep)) die("level launched before its dependency");
       const sa_pass_t& p = b.passes[w.pass];
       if (p.C * p.T > block) die("pass tile larger than the workgroup");
       if ((size_t)p.C * p.ldsstride * (prec ? 16 : 8) > lds) die("pass LDS larger than the launch");
@@ -301,8 +309,15 @@ extern "C" int sa_launch_batch(const sa_batch_t* bp, int prec, int grid, int blo
       }
       ++seen[w.pass];
     }
-    for (uint32_t i = 0; i < b.npasses; ++i)
-      if (seen[i] != b.pass_tiles[i]) die("pass not fully covered by the work list");
+    if (!b.independent) {
+      for (uint32_t i = 0; i < b.npasses; ++i)
+        if (seen[i] != b.pass_tiles[i]) die("pass not fully covered by the work list");
+    } else {
+      for (uint32_t i = 0; i < b.npasses; ++i)
+        if (seen[i] != 0 && seen[i] != b.pass_tiles[i]) die("level launch covers part of a pass");
+      for (uint32_t i = 0; i < b.npasses; ++i)
+        if (seen[i]) g_done_pass.insert((int)i);
+    }
   };
   if (g_capturing)
     g_capture->push_back(run);
