@@ -191,6 +191,154 @@ def bench_batch(args, hip):
     print(json.dumps(out), flush=True)
 
 
+def _serve_run(srv, reqs, inp, outs, paced: bool):
+    """Submit `reqs` to the server: at their Poisson arrival times (paced)
+    or all at once.  Output buffers are reused round-robin, so request i
+    waits for request i - len(outs) before it is submitted; latency is
+    counted from the scheduled arrival, so that wait shows up in it.
+    Returns the ServedJob results in request order."""
+    from shafft_amd import api
+
+    k = len(outs)
+    ids, res = [], [None] * len(reqs)
+    t0 = api.Server.now_ns() + 2_000_000
+    while time.monotonic_ns() < t0:
+        pass
+    for i, r in enumerate(reqs):
+        arr = t0 + int(r.t * 1e9) if paced else t0
+        if paced:
+            while time.monotonic_ns() < arr:
+                pass
+        if i >= k:
+            res[i - k] = srv.wait(ids[i - k])
+        o = outs[i % k]
+        if r.kind == 1:
+            ids.append(srv.submit_1d(r.n, r.prec, r.direction, inp[r.prec].ptr, o.ptr, arrival_ns=arr))
+        else:
+            ids.append(srv.submit_nd(r.dims, r.prec, r.direction, inp[r.prec].ptr, o.ptr, arrival_ns=arr))
+    for i in range(max(0, len(reqs) - k), len(reqs)):
+        res[i] = srv.wait(ids[i])
+    bad = [(reqs[i].describe(), r.message) for i, r in enumerate(res) if r.status != 0]
+    if bad:
+        raise RuntimeError(f"served jobs failed: {bad[:3]}")
+    return res
+
+
+def _lat_stats(res):
+    lat = np.sort(np.array([r.latency_ms for r in res]))
+    rank = lambda q: float(lat[max(0, int(math.ceil(q * len(lat))) - 1)])  # noqa: E731
+    span = (max(r.done for r in res) - min(r.arrival for r in res)) * 1e-9
+    return {"p50_ms": round(rank(0.5), 3), "p99_ms": round(rank(0.99), 3), "mean_ms": round(float(lat.mean()), 3),
+            "max_ms": round(float(lat[-1]), 3), "span_s": span}
+
+
+def bench_serve(args, hip):
+    """Serving stream: Poisson arrivals of FFTND/FFT1D jobs, sizes
+    log-uniform in 2^10..2^26 elements (7-smooth), 70% complex64 / 30%
+    complex128, forward or backward.  Reports the p50/p99 latency at the
+    stated arrival rate (plans built on demand, then resident), the
+    throughput with every job queued at once, and the plain per-call path
+    (the reference call sequence: one plan per shape, host-synchronous
+    execute, one job at a time) on the same jobs."""
+    from shafft_amd import api
+    from tests.serve_stream import draw_stream
+
+    reqs = draw_stream(args.requests, seed=46, rate=args.rate)
+    max_e = max(r.elems for r in reqs)
+    rng = np.random.default_rng(46)
+    inp = {}
+    for prec, dt in ((api.C2C, np.complex64), (api.Z2Z, np.complex128)):
+        x = np.empty(max_e, dt)
+        x.real = rng.uniform(-1, 1, max_e)
+        x.imag = rng.uniform(-1, 1, max_e)
+        inp[prec] = api.DeviceBuffer(max_e, prec)
+        inp[prec].upload(x)
+        del x
+    outs = [api.DeviceBuffer(max_e, api.Z2Z) for _ in range(args.outstanding)]
+    gflop = sum(r.gflop for r in reqs)
+    srv = api.Server(slots=args.slots, arena_bytes=args.arena_gb << 30, planners=2)
+    cold = _lat_stats(_serve_run(srv, reqs, inp, outs, paced=True))
+    st_cold = srv.stats()
+    srv.reset_stats()
+    warm = _lat_stats(_serve_run(srv, reqs, inp, outs, paced=True))
+    srv.reset_stats()
+    sat_res = _serve_run(srv, reqs, inp, outs, paced=False)
+    sat = _lat_stats(sat_res)
+    st = srv.stats()
+    srv.close()
+    gpu_busy_s = sum(r.gpu_ns for r in sat_res) * 1e-9
+
+    # plain per-call path on the same jobs: plans made beforehand (not timed)
+    stream = hip.stream()
+    plans = {}
+    for r in reqs:
+        key = (r.kind, r.n, tuple(r.dims or ()), r.prec)
+        if key in plans:
+            continue
+        if r.kind == 1:
+            p = api.FFT1D()
+            p.init(r.n, r.n, 0, r.prec)
+        else:
+            p = api.FFTND()
+            p.init([1] * len(r.dims), r.dims, r.prec)
+        p.plan()
+        p.set_stream(stream)
+        plans[key] = p
+    scratch = api.DeviceBuffer(max_e, api.Z2Z)
+    hip.sync()
+    t0 = time.perf_counter()
+    for r in reqs:
+        p = plans[(r.kind, r.n, tuple(r.dims or ()), r.prec)]
+        p.set_buffers(scratch.ptr, outs[0].ptr)
+        p.execute(api.BACKWARD if r.direction else api.FORWARD)  # host-synchronous
+    hip.sync()
+    plain = time.perf_counter() - t0
+    del plans
+
+    out = {
+        "metric": "served mixed-size stream throughput (GFLOP/s, 5 N log2 N per job); p50/p99 latency",
+        "value": round(gflop / sat["span_s"], 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(sat["span_s"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(reqs)} jobs, Poisson arrivals at {args.rate:g}/s, sizes log-uniform 2^10..2^26 "
+                        f"(7-smooth), 70% complex64 / 30% complex128, fwd/bwd 50/50, half FFTND 2-D/3-D, "
+                        f"half FFT1D",
+            "slots": args.slots,
+            "outstanding_buffers": args.outstanding,
+            "total_gflop": round(gflop, 2),
+            "p50_ms": warm["p50_ms"],
+            "p99_ms": warm["p99_ms"],
+            "mean_ms": warm["mean_ms"],
+            "max_ms": warm["max_ms"],
+            "paced_gflops": round(gflop / warm["span_s"], 2),
+            "cold_p50_ms": cold["p50_ms"],
+            "cold_p99_ms": cold["p99_ms"],
+            "cold_plan_builds": st_cold["plan_builds"],
+            "saturated_req_per_s": round(len(reqs) / sat["span_s"], 1),
+            "saturated_p50_ms": sat["p50_ms"],
+            "saturated_p99_ms": sat["p99_ms"],
+            "gpu_busy_frac": round(gpu_busy_s / sat["span_s"], 3),
+            "plain_per_call_s": round(plain, 4),
+            "plain_per_call_gflops": round(gflop / plain, 2),
+            "plain_per_call_req_per_s": round(len(reqs) / plain, 1),
+            "speedup_vs_plain": round(plain / sat["span_s"], 3),
+            "plans_resident": st["plans"],
+            "table_mb": round(st["table_bytes"] / 2**20, 2),
+            "arena_peak_gb": round(st["arena_peak"] / 2**30, 3),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,7 +347,12 @@ def main():
     ap.add_argument("--dims", type=str, default="512,512,512")
     ap.add_argument("--prec", type=str, default="z2z", choices=["c2c", "z2z"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="fftnd", choices=["fftnd", "batch"])
+    ap.add_argument("--workload", default="fftnd", choices=["fftnd", "batch", "serve"])
+    ap.add_argument("--requests", type=int, default=2000, help="jobs in the serve workload")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve arrival rate (jobs/s)")
+    ap.add_argument("--slots", type=int, default=8, help="server stream slots")
+    ap.add_argument("--outstanding", type=int, default=48, help="serve output buffers (max job size each)")
+    ap.add_argument("--arena-gb", type=int, default=24, help="server scratch arena (GiB)")
     ap.add_argument("--count", type=int, default=4096, help="transforms in the batch workload")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -213,6 +366,9 @@ def main():
 
     if args.workload == "batch":
         bench_batch(args, Hip())
+        return
+    if args.workload == "serve":
+        bench_serve(args, Hip())
         return
 
     from shafft_amd import api

@@ -126,6 +126,46 @@ int shafftAmdBatchInfo(shafft_amd_batch_plan* plan, int* npasses, int64_t* nitem
                        int64_t* bytes);
 int shafftAmdBatchDestroy(shafft_amd_batch_plan* plan);
 
+/* Transform server (no reference counterpart: the reference runs one
+ * synchronous transform per call, shafft_internal.cpp:62-127).  A request
+ * loop that admits FFTND (all axes, row-major dims) and FFT1D jobs of mixed
+ * shape and precision onto `slots` device streams as slots and scratch free
+ * up.  Scratch comes from one device arena of `arena_bytes` allocated at
+ * creation; a job waits until its scratch footprint fits (jobs whose
+ * footprint exceeds the arena fail with SHAFFT_ERR_ALLOC), and a later job
+ * that fits may pass an earlier waiting one at most `max_bypass` times.
+ * Plans are built on `planner_threads` host threads and stay resident.
+ * Jobs read `in` (left unchanged) and write `out`: caller-owned device
+ * buffers of the job's element count, distinct, valid until the job ends.
+ * direction: 0 forward, 1 backward; normalize != 0 scales the output by
+ * 1/sqrt(elements).  arrival_ns < 0 stamps the submit time; otherwise it is
+ * a shafftAmdServerNow() time from which latency is counted. */
+typedef struct shafft_amd_server shafft_amd_server;
+int shafftAmdServerCreate(shafft_amd_server** srv, int slots, int64_t arena_bytes,
+                          int planner_threads, int max_bypass);
+int shafftAmdServerSubmitND(shafft_amd_server* srv, int ndim, const int* dims, int precision,
+                            int direction, int normalize, const void* in, void* out,
+                            int64_t arrival_ns, int64_t* job);
+int shafftAmdServerSubmit1D(shafft_amd_server* srv, int64_t n, int precision, int direction,
+                            int normalize, const void* in, void* out, int64_t arrival_ns,
+                            int64_t* job);
+/* Waits up to timeout_ns (< 0: no limit).  *finished = 1 when the job has
+ * ended: its status goes to *job_status (0 = success; otherwise the job's
+ * message is the calling thread's last error), times[4] = arrival, admit,
+ * completion (steady-clock ns) and device ns, and the id is released. */
+int shafftAmdServerWait(shafft_amd_server* srv, int64_t job, int64_t timeout_ns, int* finished,
+                        int* job_status, int64_t* times);
+int shafftAmdServerQuery(shafft_amd_server* srv, int64_t job, int* finished);
+int shafftAmdServerDrain(shafft_amd_server* srv);
+/* counts[8] = submitted, completed, failed, plans, plan builds, arena bytes,
+ * arena peak bytes, resident table bytes; values[6] = p50, p99, mean, max
+ * latency (ms), span from first arrival to last completion (s), GFLOP
+ * completed (5 N log2 N). */
+int shafftAmdServerStats(shafft_amd_server* srv, int64_t* counts, double* values);
+int shafftAmdServerResetStats(shafft_amd_server* srv);
+int shafftAmdServerNow(int64_t* ns);
+int shafftAmdServerDestroy(shafft_amd_server* srv); /* drains first */
+
 /* ======================================================================== */
 /* 2. Reference C API (include/shafft/shafft.h) at one rank per GPU          */
 /* ======================================================================== */

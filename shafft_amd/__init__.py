@@ -19,6 +19,8 @@ from .api import (  # noqa: F401
     FFTND,
     NDBackendPlan,
     OneDBackendPlan,
+    Server,
+    ServedJob,
     ShafftError,
     lib,
     library_path,

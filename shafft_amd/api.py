@@ -69,6 +69,19 @@ _SIGS = {
     "shafftAmdBatchInfo": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int),
                                       ct.POINTER(ct.c_int64)]),
     "shafftAmdBatchDestroy": (ct.c_int, [ct.c_void_p]),
+    "shafftAmdServerCreate": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.c_int64, ct.c_int, ct.c_int]),
+    "shafftAmdServerSubmitND": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int), ct.c_int, ct.c_int, ct.c_int,
+                                           ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int64)]),
+    "shafftAmdServerSubmit1D": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
+                                           ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int64)]),
+    "shafftAmdServerWait": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, ct.POINTER(ct.c_int),
+                                       ct.POINTER(ct.c_int), ct.POINTER(ct.c_int64)]),
+    "shafftAmdServerQuery": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_int)]),
+    "shafftAmdServerDrain": (ct.c_int, [ct.c_void_p]),
+    "shafftAmdServerStats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_double)]),
+    "shafftAmdServerResetStats": (ct.c_int, [ct.c_void_p]),
+    "shafftAmdServerNow": (ct.c_int, [ct.POINTER(ct.c_int64)]),
+    "shafftAmdServerDestroy": (ct.c_int, [ct.c_void_p]),
     # reference C API
     "shafftSetStream": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "shafftLastErrorStatus": (ct.c_int, []),
@@ -430,5 +443,101 @@ class BatchPlan:
             if self.h.value:
                 self.L.shafftAmdBatchDestroy(self.h)
                 self.h = ct.c_void_p()
+        except Exception:
+            pass
+
+
+class ServedJob:
+    """Outcome of one served job: status (0 = success), error message, and
+    host steady-clock times in ns (arrival, admit, done) plus device ns."""
+
+    def __init__(self, status, message, times):
+        self.status = status
+        self.message = message
+        self.arrival, self.admit, self.done, self.gpu_ns = (int(t) for t in times)
+
+    @property
+    def latency_ms(self) -> float:
+        return (self.done - self.arrival) * 1e-6
+
+
+class Server:
+    """Continuous-batching transform server (shafftAmdServer* in
+    include/shafft_amd.h): jobs of mixed shape and precision are admitted
+    onto `slots` device streams as slots and arena scratch free up."""
+
+    def __init__(self, slots: int = 8, arena_bytes: int = 16 << 30, planners: int = 2, max_bypass: int = 64,
+                 L=None):
+        self.L = L or _L()
+        self.h = ct.c_void_p()
+        check(self.L.shafftAmdServerCreate(ct.byref(self.h), slots, arena_bytes, planners, max_bypass),
+              "shafftAmdServerCreate", self.L)
+
+    @staticmethod
+    def now_ns(L=None) -> int:
+        L = L or _L()
+        v = ct.c_int64()
+        check(L.shafftAmdServerNow(ct.byref(v)), "shafftAmdServerNow", L)
+        return v.value
+
+    def submit_nd(self, dims, prec: int, direction: int, src: int, dst: int, normalize: bool = False,
+                  arrival_ns: int = -1) -> int:
+        dd = (ct.c_int * len(dims))(*[int(d) for d in dims])
+        job = ct.c_int64()
+        check(self.L.shafftAmdServerSubmitND(self.h, len(dims), dd, prec, direction, 1 if normalize else 0,
+                                             ct.c_void_p(src), ct.c_void_p(dst), arrival_ns, ct.byref(job)),
+              "shafftAmdServerSubmitND", self.L)
+        return job.value
+
+    def submit_1d(self, n: int, prec: int, direction: int, src: int, dst: int, normalize: bool = False,
+                  arrival_ns: int = -1) -> int:
+        job = ct.c_int64()
+        check(self.L.shafftAmdServerSubmit1D(self.h, int(n), prec, direction, 1 if normalize else 0,
+                                             ct.c_void_p(src), ct.c_void_p(dst), arrival_ns, ct.byref(job)),
+              "shafftAmdServerSubmit1D", self.L)
+        return job.value
+
+    def wait(self, job: int, timeout_s: float = -1.0):
+        """ServedJob once the job has ended, None on timeout."""
+        fin, st = ct.c_int(), ct.c_int()
+        times = (ct.c_int64 * 4)()
+        tns = -1 if timeout_s < 0 else int(timeout_s * 1e9)
+        check(self.L.shafftAmdServerWait(self.h, job, tns, ct.byref(fin), ct.byref(st), times),
+              "shafftAmdServerWait", self.L)
+        if not fin.value:
+            return None
+        msg = last_error_message(self.L) if st.value else ""
+        return ServedJob(st.value, msg, list(times))
+
+    def finished(self, job: int) -> bool:
+        fin = ct.c_int()
+        check(self.L.shafftAmdServerQuery(self.h, job, ct.byref(fin)), "shafftAmdServerQuery", self.L)
+        return bool(fin.value)
+
+    def drain(self):
+        check(self.L.shafftAmdServerDrain(self.h), "shafftAmdServerDrain", self.L)
+
+    def stats(self) -> dict:
+        c = (ct.c_int64 * 8)()
+        v = (ct.c_double * 6)()
+        check(self.L.shafftAmdServerStats(self.h, c, v), "shafftAmdServerStats", self.L)
+        keys_c = ("submitted", "completed", "failed", "plans", "plan_builds", "arena_bytes", "arena_peak",
+                  "table_bytes")
+        keys_v = ("p50_ms", "p99_ms", "mean_ms", "max_ms", "span_s", "gflop")
+        out = {k: int(x) for k, x in zip(keys_c, c)}
+        out.update({k: float(x) for k, x in zip(keys_v, v)})
+        return out
+
+    def reset_stats(self):
+        check(self.L.shafftAmdServerResetStats(self.h), "shafftAmdServerResetStats", self.L)
+
+    def close(self):
+        if self.h.value:
+            check(self.L.shafftAmdServerDestroy(self.h), "shafftAmdServerDestroy", self.L)
+            self.h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

@@ -41,6 +41,7 @@ NDBackend::NDBackend(int nta, const int* ta, int ndim, const int* dims, int prec
   }
   map_pp_ = prog_.assign(PassProgram::PINGPONG, elems_);
   map_ip_ = prog_.assign(PassProgram::INPLACE, elems_);
+  map_pr_ = prog_.assign(PassProgram::PRESERVE, elems_);
   ensure_scratch(map_pp_, prec_, s1_, s2_);
   if (diag_enabled()) {
     std::ostringstream os;
@@ -61,6 +62,28 @@ void NDBackend::execute(void*& data, void*& work, int dir) {
   if (!inplace && m.result == SLOT_WORK) std::swap(data, work);
 }
 
+// Shared by both backends: run a PRESERVE-mode program in -> out.
+static void run_preserve(const PassProgram& prog, const PassProgram::SlotMap& m, int64_t elems,
+                         int prec, const void* in, void* out, void* s1, void* s2, int dir,
+                         hipStream_t stream) {
+  if (!in || !out) fail(ERR_NO_BUFFER, "buffer not set");
+  if (in == out) fail(ERR_INVALID_LAYOUT, "input-preserving execute needs in != out");
+  if ((m.scratch[0] && !s1) || (m.scratch[1] && !s2)) fail(ERR_NO_BUFFER, "scratch not set");
+  if (prog.npasses() == 0) {
+    hip_check(hipMemcpyAsync(out, in, (size_t)elems * (prec ? 16 : 8), hipMemcpyDeviceToDevice,
+                             stream),
+              "hipMemcpyAsync");
+    return;
+  }
+  void* slots[SLOT_COUNT] = {const_cast<void*>(in), out, s1, s2};
+  prog.run(m, slots, dir, stream);
+}
+
+void NDBackend::execute_preserve(const void* in, void* out, void* s1, void* s2, int dir,
+                                 hipStream_t stream) const {
+  run_preserve(prog_, map_pr_, elems_, prec_, in, out, s1, s2, dir, stream);
+}
+
 void NDBackend::execute_pass(size_t i, void* in, void* out, int dir) {
   if (i >= prog_.npasses()) fail(ERR_INVALID_DIM, "pass index out of range");
   PassProgram::SlotMap m;
@@ -70,13 +93,14 @@ void NDBackend::execute_pass(size_t i, void* in, void* out, int dir) {
   prog_.run_one(m, i, slots, dir, stream_);
 }
 
-OneDBackend::OneDBackend(int64_t n, int prec) : n_(n), prec_(prec), prog_(prec) {
+OneDBackend::OneDBackend(int64_t n, int prec, bool own_scratch)
+    : n_(n), prec_(prec), prog_(prec) {
   if (prec != 0 && prec != 1) fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
   if (n < 1) fail(ERR_INVALID_DIM, "invalid length");
   prog_.add_transform(n, std::vector<Digit>{}, 1, 1);
   map_pr_ = prog_.assign(PassProgram::PRESERVE, n_);
   map_ip_ = prog_.assign(PassProgram::INPLACE, n_);
-  ensure_scratch(map_pr_, prec_, s1_, s2_);
+  if (own_scratch) ensure_scratch(map_pr_, prec_, s1_, s2_);
   if (diag_enabled()) {
     std::ostringstream os;
     os << "N=" << n << " prec=" << (prec ? "Z2Z" : "C2C") << " passes=\n" << prog_.describe();
@@ -88,7 +112,7 @@ void OneDBackend::execute(void* in, void* out, int dir) {
   if (!in || !out) fail(ERR_NO_BUFFER, "buffer not set");
   const bool inplace = in == out;
   const PassProgram::SlotMap& m = inplace ? map_ip_ : map_pr_;
-  if (inplace) ensure_scratch(m, prec_, s1_, s2_);
+  ensure_scratch(m, prec_, s1_, s2_);
   if (prog_.npasses() == 0) {
     if (!inplace)
       hip_check(hipMemcpyAsync(out, in, (size_t)n_ * (prec_ ? 16 : 8), hipMemcpyDeviceToDevice,
@@ -98,6 +122,11 @@ void OneDBackend::execute(void* in, void* out, int dir) {
   }
   void* slots[SLOT_COUNT] = {in, out, s1_.p, s2_.p};
   prog_.run(m, slots, dir, stream_);
+}
+
+void OneDBackend::execute_preserve(const void* in, void* out, void* s1, void* s2, int dir,
+                                   hipStream_t stream) const {
+  run_preserve(prog_, map_pr_, n_, prec_, in, out, s1, s2, dir, stream);
 }
 
 void OneDBackend::execute_pass(size_t i, void* in, void* out, int dir) {

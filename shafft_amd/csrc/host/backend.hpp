@@ -28,12 +28,19 @@ class NDBackend {
   const PassProgram& program() const { return prog_; }
   // Profiling aid: run pass i alone from `in` to `out` (asynchronous).
   void execute_pass(size_t i, void* in, void* out, int dir);
+  // in -> out with `in` preserved, asynchronous on `stream`, using the
+  // caller's scratch s1/s2 (preserve_scratch_elems(0/1) elements).  The plan
+  // holds no per-call state, so calls on different streams may overlap.
+  void execute_preserve(const void* in, void* out, void* s1, void* s2, int dir,
+                        hipStream_t stream) const;
+  int64_t preserve_scratch_elems(int i) const { return map_pr_.scratch[i]; }
+  int64_t table_bytes() const { return prog_.table_bytes(); }
 
  private:
   int prec_;
   int64_t elems_ = 1;
   PassProgram prog_;
-  PassProgram::SlotMap map_pp_, map_ip_;
+  PassProgram::SlotMap map_pp_, map_ip_, map_pr_;
   DevBuf s1_, s2_;
   hipStream_t stream_ = nullptr;
 };
@@ -42,14 +49,20 @@ class NDBackend {
 // backend; fft1d_method.hip:416-650).  Input is preserved.
 class OneDBackend {
  public:
-  OneDBackend(int64_t n, int prec);
+  // own_scratch=false leaves scratch to the caller (execute_preserve only).
+  OneDBackend(int64_t n, int prec, bool own_scratch = true);
   void execute(void* in, void* out, int dir);
+  void execute_preserve(const void* in, void* out, void* s1, void* s2, int dir,
+                        hipStream_t stream) const;
+  int64_t preserve_scratch_elems(int i) const { return map_pr_.scratch[i]; }
+  int64_t table_bytes() const { return prog_.table_bytes(); }
   void set_stream(hipStream_t s) { stream_ = s; }
   hipStream_t stream() const { return stream_; }
   int prec() const { return prec_; }
   int64_t n() const { return n_; }
   const PassProgram& program() const { return prog_; }
   void execute_pass(size_t i, void* in, void* out, int dir);
+  int64_t scratch_bytes() const { return (int64_t)(s1_.bytes + s2_.bytes); }
 
  private:
   int64_t n_;

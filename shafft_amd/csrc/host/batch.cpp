@@ -9,26 +9,24 @@
 namespace sa {
 
 static int cu_count() {
-  static int cached = -1;
-  if (cached < 0) {
+  static const int cached = [] {
     int dev = 0, v = 0;
-    cached = 256;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-      cached = v;
-  }
+      return v;
+    return 256;
+  }();
   return cached;
 }
 
 static uint64_t wall_clock_hz() {
-  static uint64_t cached = 0;
-  if (!cached) {
+  static const uint64_t cached = []() -> uint64_t {
     int dev = 0, khz = 0;
-    cached = 100000000ull;  // gfx9 wall clock: 100 MHz
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
-      cached = (uint64_t)khz * 1000ull;
-  }
+      return (uint64_t)khz * 1000ull;
+    return (uint64_t)100000000ull;  // gfx9 wall clock: 100 MHz
+  }();
   return cached;
 }
 

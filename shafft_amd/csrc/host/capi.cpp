@@ -18,6 +18,7 @@
 #include "backend.hpp"
 #include "batch.hpp"
 #include "common.hpp"
+#include "server.hpp"
 
 using namespace sa;
 
@@ -33,6 +34,9 @@ struct shafft_amd_1d_plan {
 };
 struct shafft_amd_batch_plan {
   std::unique_ptr<BatchPlan> be;
+};
+struct shafft_amd_server {
+  std::unique_ptr<Server> be;
 };
 
 namespace {
@@ -366,6 +370,136 @@ int shafftAmdBatchInfo(shafft_amd_batch_plan* plan, int* npasses, int64_t* nitem
 int shafftAmdBatchDestroy(shafft_amd_batch_plan* plan) {
   return guarded([&] {
     delete plan;
+    return 0;
+  });
+}
+
+// Transform server -----------------------------------------------------------
+int shafftAmdServerCreate(shafft_amd_server** srv, int slots, int64_t arena_bytes,
+                          int planner_threads, int max_bypass) {
+  return guarded([&] {
+    if (!srv) fail(ERR_NULLPTR, "null server pointer");
+    *srv = nullptr;
+    Server::Config cfg;
+    cfg.slots = slots;
+    cfg.arena_bytes = arena_bytes;
+    cfg.planners = planner_threads;
+    cfg.max_bypass = max_bypass;
+    auto p = std::make_unique<shafft_amd_server>();
+    p->be = std::make_unique<Server>(cfg);
+    *srv = p.release();
+    return 0;
+  });
+}
+
+int shafftAmdServerSubmitND(shafft_amd_server* srv, int ndim, const int* dims, int precision,
+                            int direction, int normalize, const void* in, void* out,
+                            int64_t arrival_ns, int64_t* job) {
+  return guarded([&] {
+    if (!srv || !dims || !job) fail(ERR_NULLPTR, "null argument");
+    if (ndim < 1 || ndim > 8) fail(ERR_INVALID_DIM, "ndim must be in [1, 8]");
+    ServeJob d;
+    d.kind = 0;
+    d.ndim = ndim;
+    for (int i = 0; i < ndim; ++i) d.dims[i] = dims[i];
+    d.prec = precision;
+    d.dir = direction;
+    d.normalize = normalize != 0;
+    *job = srv->be->submit(d, in, out, arrival_ns);
+    return 0;
+  });
+}
+
+int shafftAmdServerSubmit1D(shafft_amd_server* srv, int64_t n, int precision, int direction,
+                            int normalize, const void* in, void* out, int64_t arrival_ns,
+                            int64_t* job) {
+  return guarded([&] {
+    if (!srv || !job) fail(ERR_NULLPTR, "null argument");
+    ServeJob d;
+    d.kind = 1;
+    d.n = n;
+    d.prec = precision;
+    d.dir = direction;
+    d.normalize = normalize != 0;
+    *job = srv->be->submit(d, in, out, arrival_ns);
+    return 0;
+  });
+}
+
+int shafftAmdServerWait(shafft_amd_server* srv, int64_t job, int64_t timeout_ns, int* finished,
+                        int* job_status, int64_t* times) {
+  return guarded([&] {
+    if (!srv || !finished) fail(ERR_NULLPTR, "null argument");
+    int st = 0;
+    ServeTimes t;
+    std::string msg;
+    *finished = srv->be->wait(job, timeout_ns, &st, &t, &msg) == 0;
+    if (*finished) {
+      if (job_status) *job_status = st;
+      if (times) {
+        times[0] = t.arrival;
+        times[1] = t.admit;
+        times[2] = t.done;
+        times[3] = t.gpu_ns;
+      }
+      if (st) set_last_error(st, SRC_NONE, 0, msg.c_str());
+    }
+    return 0;
+  });
+}
+
+int shafftAmdServerQuery(shafft_amd_server* srv, int64_t job, int* finished) {
+  return guarded([&] {
+    if (!srv || !finished) fail(ERR_NULLPTR, "null argument");
+    *finished = srv->be->query(job) == 0;
+    return 0;
+  });
+}
+
+int shafftAmdServerDrain(shafft_amd_server* srv) {
+  return guarded([&] {
+    if (!srv) fail(ERR_NULLPTR, "null server");
+    srv->be->drain();
+    return 0;
+  });
+}
+
+int shafftAmdServerStats(shafft_amd_server* srv, int64_t* counts, double* values) {
+  return guarded([&] {
+    if (!srv) fail(ERR_NULLPTR, "null server");
+    const ServeStats s = srv->be->stats();
+    if (counts) {
+      const int64_t c[8] = {s.submitted, s.completed,   s.failed,      s.plans,
+                            s.plan_builds, s.arena_bytes, s.arena_peak, s.table_bytes};
+      for (int i = 0; i < 8; ++i) counts[i] = c[i];
+    }
+    if (values) {
+      const double v[6] = {s.p50_ms, s.p99_ms, s.mean_ms, s.max_ms, s.span_s, s.gflop};
+      for (int i = 0; i < 6; ++i) values[i] = v[i];
+    }
+    return 0;
+  });
+}
+
+int shafftAmdServerResetStats(shafft_amd_server* srv) {
+  return guarded([&] {
+    if (!srv) fail(ERR_NULLPTR, "null server");
+    srv->be->reset_stats();
+    return 0;
+  });
+}
+
+int shafftAmdServerNow(int64_t* ns) {
+  return guarded([&] {
+    if (!ns) fail(ERR_NULLPTR, "null argument");
+    *ns = Server::now_ns();
+    return 0;
+  });
+}
+
+int shafftAmdServerDestroy(shafft_amd_server* srv) {
+  return guarded([&] {
+    delete srv;
     return 0;
   });
 }

@@ -30,16 +30,17 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
 }
 
 // ---------------------------------------------------------------------------
+// Device queries are made once per process (thread-safe static init): plans
+// may be built on several host threads at once.
 static int lds_budget_bytes() {
-  static int cached = -1;
-  if (cached < 0) {
+  static const int cached = [] {
     int dev = 0, v = 0;
-    cached = 64 * 1024;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
         v > 0)
-      cached = std::min(v, sa_max_lds_bytes());
-  }
+      return std::min(v, sa_max_lds_bytes());
+    return 64 * 1024;
+  }();
   return cached;
 }
 
@@ -242,14 +243,13 @@ static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
 }
 
 static int device_cus() {
-  static int cached = -1;
-  if (cached < 0) {
+  static const int cached = [] {
     int dev = 0, v = 0;
-    cached = 256;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-      cached = v;
-  }
+      return v;
+    return 256;
+  }();
   return cached;
 }
 
@@ -674,6 +674,12 @@ void PassProgram::run_one(const SlotMap& m, size_t i, void* const slots[SLOT_COU
 void PassProgram::run(const SlotMap& m, void* const slots[SLOT_COUNT], int dir,
                       hipStream_t stream) const {
   for (size_t i = 0; i < passes_.size(); ++i) run_one(m, i, slots, dir, stream);
+}
+
+int64_t PassProgram::table_bytes() const {
+  int64_t b = 0;
+  for (auto& t : tables_) b += (int64_t)t.bytes;
+  return b;
 }
 
 int64_t PassProgram::total_bytes() const {

@@ -118,6 +118,7 @@ class PassProgram {
   PassSpec& pass_mut(size_t i) { return passes_[i]; }
   const PassSpec& pass(size_t i) const { return passes_[i]; }
   int64_t total_bytes() const;
+  int64_t table_bytes() const;  // device bytes of the plan's tables
   std::string describe() const;
   int precision() const { return prec_; }
 

@@ -1,0 +1,462 @@
+#include "server.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <sstream>
+
+namespace sa {
+
+int64_t ServeJob::elems() const {
+  if (kind == 1) return n;
+  int64_t e = 1;
+  for (int i = 0; i < ndim; ++i) e *= dims[i];
+  return e;
+}
+
+std::string ServeJob::key() const {
+  std::ostringstream os;
+  os << (kind ? "1d" : "nd") << (prec ? "z" : "c");
+  if (kind)
+    os << ":" << n;
+  else
+    for (int i = 0; i < ndim; ++i) os << ":" << dims[i];
+  return os.str();
+}
+
+// ---------------------------------------------------------------------------
+Arena::Arena(int64_t bytes) : buf_((size_t)round(std::max<int64_t>(bytes, 0))) {
+  size_ = (int64_t)buf_.bytes;
+  if (size_ > 0) free_[0] = size_;
+}
+
+int64_t Arena::alloc(int64_t bytes) {
+  const int64_t need = round(bytes);
+  if (need == 0) return 0;
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second < need) continue;
+    const int64_t off = it->first, rest = it->second - need;
+    free_.erase(it);
+    if (rest > 0) free_[off + need] = rest;
+    used_ += need;
+    peak_ = std::max(peak_, used_);
+    return off;
+  }
+  return -1;
+}
+
+void Arena::release(int64_t off, int64_t bytes) {
+  const int64_t len = round(bytes);
+  if (len == 0) return;
+  used_ -= len;
+  auto it = free_.emplace(off, len).first;
+  auto nx = std::next(it);
+  if (nx != free_.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    free_.erase(nx);
+  }
+  if (it != free_.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      free_.erase(it);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+int64_t Server::now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+Server::Server(const Config& cfg) : cfg_(cfg), arena_(cfg.arena_bytes) {
+  if (cfg.slots < 1 || cfg.slots > 64) fail(ERR_INVALID_DIM, "slots must be in [1, 64]");
+  if (cfg.planners < 1 || cfg.planners > 16) fail(ERR_INVALID_DIM, "planners must be in [1, 16]");
+  if (cfg.max_bypass < 0) fail(ERR_INVALID_DIM, "max_bypass must be >= 0");
+  hip_check(hipGetDevice(&device_), "hipGetDevice");
+  slots_.resize(cfg.slots);
+  try {
+    for (Slot& s : slots_) {
+      // Default priority; non-blocking so host-side table uploads on the
+      // null stream never wait for served work.
+      hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipEventCreate(&s.start), "hipEventCreate");
+      hip_check(hipEventCreate(&s.end), "hipEventCreate");
+    }
+  } catch (...) {
+    destroy_slots();
+    throw;
+  }
+  for (int i = 0; i < cfg.planners; ++i) planners_.emplace_back([this] { planner(); });
+  sched_ = std::thread([this] { scheduler(); });
+}
+
+void Server::destroy_slots() {
+  for (Slot& s : slots_) {
+    if (s.start) (void)hipEventDestroy(s.start);
+    if (s.end) (void)hipEventDestroy(s.end);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    s = Slot{};
+  }
+}
+
+Server::~Server() {
+  drain();
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  sched_cv_.notify_all();
+  plan_cv_.notify_all();
+  if (sched_.joinable()) sched_.join();
+  for (auto& t : planners_)
+    if (t.joinable()) t.join();
+  for (Slot& s : slots_)
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+  destroy_slots();
+}
+
+int64_t Server::submit(const ServeJob& d, const void* in, void* out, int64_t arrival_ns) {
+  if (d.kind != 0 && d.kind != 1) fail(ERR_INVALID_DIM, "job kind must be 0 (FFTND) or 1 (FFT1D)");
+  if (d.prec != 0 && d.prec != 1) fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
+  if (d.dir != 0 && d.dir != 1) fail(ERR_INVALID_DIM, "direction must be 0 or 1");
+  if (d.kind == 0) {
+    if (d.ndim < 1 || d.ndim > 8) fail(ERR_INVALID_DIM, "ndim must be in [1, 8]");
+    for (int i = 0; i < d.ndim; ++i)
+      if (d.dims[i] <= 0) fail(ERR_INVALID_DIM, "dimension must be positive");
+    int64_t e = 1;
+    for (int i = 0; i < d.ndim; ++i) {
+      if (e > INT64_MAX / 16 / d.dims[i]) fail(ERR_SIZE_OVERFLOW, "job size overflows");
+      e *= d.dims[i];
+    }
+  } else if (d.n < 1 || d.n > INT64_MAX / 16) {
+    fail(ERR_INVALID_DIM, "invalid FFT1D length");
+  }
+  if (!in || !out) fail(ERR_NULLPTR, "null job buffer");
+  if (in == out) fail(ERR_INVALID_LAYOUT, "served jobs need distinct input and output buffers");
+  const int64_t now = now_ns();
+  std::lock_guard<std::mutex> lk(mu_);
+  if (stop_) fail(ERR_INVALID_STATE, "server is shutting down");
+  const int64_t id = next_id_++;
+  Job& j = jobs_[id];
+  j.d = d;
+  j.in = in;
+  j.out = out;
+  j.t.arrival = arrival_ns < 0 ? now : arrival_ns;
+  const std::string key = d.key();
+  auto it = plans_.find(key);
+  if (it == plans_.end()) {
+    auto p = std::make_unique<Plan>();
+    p->desc = d;
+    j.plan = p.get();
+    plans_.emplace(key, std::move(p));
+    plan_queue_.push_back(key);
+    plan_cv_.notify_one();
+  } else {
+    j.plan = it->second.get();
+  }
+  pending_.push_back(id);
+  ++n_submitted_;
+  dirty_ = true;
+  sched_cv_.notify_one();
+  return id;
+}
+
+void Server::planner() {
+  (void)hipSetDevice(device_);
+  std::unique_lock<std::mutex> lk(mu_);
+  for (;;) {
+    plan_cv_.wait(lk, [&] { return stop_ || !plan_queue_.empty(); });
+    if (plan_queue_.empty()) return;  // stop_ and nothing left to build
+    const std::string key = plan_queue_.front();
+    plan_queue_.pop_front();
+    Plan* p = plans_.at(key).get();
+    const ServeJob d = p->desc;
+    lk.unlock();
+    std::unique_ptr<NDBackend> nd;
+    std::unique_ptr<OneDBackend> od;
+    int64_t scratch[2] = {0, 0};
+    int status = 0;
+    std::string msg;
+    try {
+      const int64_t esz = d.prec ? 16 : 8;
+      if (d.kind == 0) {
+        std::vector<int> ta(d.ndim);
+        for (int i = 0; i < d.ndim; ++i) ta[i] = i;
+        nd = std::make_unique<NDBackend>(d.ndim, ta.data(), d.ndim, d.dims, d.prec);
+        for (int i = 0; i < 2; ++i) scratch[i] = nd->preserve_scratch_elems(i) * esz;
+      } else {
+        od = std::make_unique<OneDBackend>(d.n, d.prec, false);
+        for (int i = 0; i < 2; ++i) scratch[i] = od->preserve_scratch_elems(i) * esz;
+      }
+    } catch (const Error& e) {
+      status = e.status;
+      msg = e.what();
+    } catch (const std::exception& e) {
+      status = ERR_INTERNAL;
+      msg = e.what();
+    }
+    lk.lock();
+    p->nd = std::move(nd);
+    p->od = std::move(od);
+    p->scratch[0] = scratch[0];
+    p->scratch[1] = scratch[1];
+    p->status = status;
+    p->msg = msg;
+    p->state = status ? Plan::FAILED : Plan::READY;
+    ++n_builds_;
+    dirty_ = true;
+    sched_cv_.notify_one();
+  }
+}
+
+void Server::release(Job& j) {
+  for (int i = 0; i < 2; ++i)
+    if (j.off[i] >= 0) {
+      arena_.release(j.off[i], j.bytes[i]);
+      j.off[i] = -1;
+    }
+  if (j.slot >= 0) {
+    slots_[j.slot].job = -1;
+    j.slot = -1;
+    --running_;
+  }
+}
+
+void Server::finish(Job& j, int status, const std::string& msg) {
+  if (j.state == RUNNING) release(j);
+  j.state = DONE;
+  j.status = status;
+  j.msg = msg;
+  if (!j.t.done) j.t.done = now_ns();
+  if (status == 0) {
+    ++n_completed_;
+    lat_ns_.push_back(j.t.done - j.t.arrival);
+    first_arrival_ = std::min(first_arrival_, j.t.arrival);
+    last_done_ = std::max(last_done_, j.t.done);
+    const double n = (double)j.d.elems();
+    if (n > 1) gflop_ += 5.0 * n * std::log2(n) * 1e-9;
+  } else {
+    ++n_failed_;
+  }
+  dirty_ = true;
+  done_cv_.notify_all();
+}
+
+// Launch one admitted job on its slot (scheduler thread, lock released).
+// Returns 0 or the failure status (message in msg).
+int Server::launch(Job& j, const Plan& p, Slot& s, std::string& msg) {
+  try {
+    void* s1 = j.off[0] >= 0 ? arena_.base() + j.off[0] : nullptr;
+    void* s2 = j.off[1] >= 0 ? arena_.base() + j.off[1] : nullptr;
+    const int dir = j.d.dir == 0 ? -1 : +1;
+    hip_check(hipEventRecord(s.start, s.stream), "hipEventRecord");
+    if (p.nd)
+      p.nd->execute_preserve(j.in, j.out, s1, s2, dir, s.stream);
+    else
+      p.od->execute_preserve(j.in, j.out, s1, s2, dir, s.stream);
+    if (j.d.normalize) {
+      const long double f = 1.0L / std::sqrt((long double)j.d.elems());
+      scale_async(j.out, (double)f, (uint64_t)j.d.elems(), j.d.prec, s.stream);
+    }
+    hip_check(hipEventRecord(s.end, s.stream), "hipEventRecord");
+    return 0;
+  } catch (const Error& e) {
+    msg = e.what();
+    (void)hipStreamSynchronize(s.stream);  // nothing may still use the scratch
+    return e.status;
+  } catch (const std::exception& e) {
+    msg = e.what();
+    (void)hipStreamSynchronize(s.stream);
+    return ERR_INTERNAL;
+  }
+}
+
+void Server::scheduler() {
+  (void)hipSetDevice(device_);
+  std::unique_lock<std::mutex> lk(mu_);
+  std::vector<int64_t> admitted;
+  for (;;) {
+    bool progress = false;
+    // Retire: a slot's job is done when its end event has completed.
+    for (Slot& s : slots_) {
+      if (s.job < 0 || s.launching) continue;
+      const hipError_t e = hipEventQuery(s.end);
+      if (e == hipErrorNotReady) continue;
+      Job& j = jobs_.at(s.job);
+      j.t.done = now_ns();
+      if (e == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, s.start, s.end) == hipSuccess)
+          j.t.gpu_ns = (int64_t)((double)ms * 1e6);
+        finish(j, 0, std::string());
+      } else {
+        finish(j, ERR_BACKEND, std::string("served job: ") + hipGetErrorString(e));
+      }
+      progress = true;
+    }
+    // Admit in arrival order while slots and arena space last.
+    admitted.clear();
+    if (dirty_) {
+      dirty_ = false;
+      std::vector<Job*> blocked;
+      int free_slots = 0;
+      for (const Slot& s : slots_) free_slots += s.job < 0;
+      for (auto it = pending_.begin(); it != pending_.end() && free_slots > 0;) {
+        Job& j = jobs_.at(*it);
+        const Plan& p = *j.plan;
+        if (p.state == Plan::BUILDING) {
+          ++it;
+          continue;
+        }
+        if (p.state == Plan::FAILED) {
+          finish(j, p.status, p.msg);
+          it = pending_.erase(it);
+          continue;
+        }
+        if (Arena::round(p.scratch[0]) + Arena::round(p.scratch[1]) > arena_.size()) {
+          finish(j, ERR_ALLOC, "job scratch (" + std::to_string(p.scratch[0] + p.scratch[1]) +
+                                   " bytes) exceeds the server arena");
+          it = pending_.erase(it);
+          continue;
+        }
+        bool hol = false;  // an earlier job has been passed too often
+        for (const Job* b : blocked) hol |= b->bypassed >= cfg_.max_bypass;
+        if (hol) break;
+        int64_t off0 = arena_.alloc(p.scratch[0]), off1 = -1;
+        if (off0 >= 0) {
+          off1 = arena_.alloc(p.scratch[1]);
+          if (off1 < 0) {
+            arena_.release(off0, p.scratch[0]);
+            off0 = -1;
+          }
+        }
+        if (off0 < 0) {
+          blocked.push_back(&j);
+          ++it;
+          continue;
+        }
+        for (Job* b : blocked) ++b->bypassed;
+        int si = 0;
+        while (slots_[si].job >= 0) ++si;
+        slots_[si].job = *it;
+        slots_[si].launching = true;
+        j.slot = si;
+        j.off[0] = p.scratch[0] ? off0 : -1;
+        j.off[1] = p.scratch[1] ? off1 : -1;
+        j.bytes[0] = p.scratch[0];
+        j.bytes[1] = p.scratch[1];
+        j.state = RUNNING;
+        j.t.admit = now_ns();
+        ++running_;
+        --free_slots;
+        admitted.push_back(*it);
+        it = pending_.erase(it);
+      }
+      progress = true;
+    }
+    if (!admitted.empty()) {
+      // Job and plan objects are node-stable and only this thread retires
+      // running jobs, so the references stay valid without the lock.
+      std::vector<std::pair<int, std::string>> res(admitted.size());
+      std::vector<Job*> js(admitted.size());
+      for (size_t k = 0; k < admitted.size(); ++k) js[k] = &jobs_.at(admitted[k]);
+      lk.unlock();
+      for (size_t k = 0; k < admitted.size(); ++k)
+        res[k].first = launch(*js[k], *js[k]->plan, slots_[js[k]->slot], res[k].second);
+      lk.lock();
+      for (size_t k = 0; k < admitted.size(); ++k) {
+        slots_[js[k]->slot].launching = false;
+        if (res[k].first) finish(*js[k], res[k].first, res[k].second);
+      }
+    }
+    if (stop_ && running_ == 0 && pending_.empty()) break;
+    if (!progress) {
+      if (running_ > 0) {
+        lk.unlock();
+        std::this_thread::yield();
+        lk.lock();
+      } else {
+        sched_cv_.wait(lk, [&] { return stop_ || dirty_; });
+      }
+    }
+  }
+}
+
+int Server::wait(int64_t id, int64_t timeout_ns, int* status, ServeTimes* t, std::string* msg) {
+  std::unique_lock<std::mutex> lk(mu_);
+  auto it = jobs_.find(id);
+  if (it == jobs_.end()) fail(ERR_INVALID_STATE, "unknown job id " + std::to_string(id));
+  Job& j = it->second;
+  auto ready = [&] { return j.state == DONE; };
+  if (timeout_ns < 0) {
+    done_cv_.wait(lk, ready);
+  } else if (!done_cv_.wait_for(lk, std::chrono::nanoseconds(timeout_ns), ready)) {
+    return 1;
+  }
+  if (status) *status = j.status;
+  if (t) *t = j.t;
+  if (msg) *msg = j.msg;
+  jobs_.erase(it);
+  return 0;
+}
+
+int Server::query(int64_t id) const {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = jobs_.find(id);
+  if (it == jobs_.end()) fail(ERR_INVALID_STATE, "unknown job id " + std::to_string(id));
+  return it->second.state == DONE ? 0 : 1;
+}
+
+void Server::drain() {
+  std::unique_lock<std::mutex> lk(mu_);
+  done_cv_.wait(lk, [&] { return pending_.empty() && running_ == 0; });
+}
+
+ServeStats Server::stats() const {
+  std::lock_guard<std::mutex> lk(mu_);
+  ServeStats s;
+  s.submitted = n_submitted_;
+  s.completed = n_completed_;
+  s.failed = n_failed_;
+  s.plans = (int64_t)plans_.size();
+  s.plan_builds = n_builds_;
+  s.arena_bytes = arena_.size();
+  s.arena_peak = arena_.peak();
+  for (const auto& kv : plans_) {
+    const Plan& p = *kv.second;
+    if (p.state != Plan::READY) continue;
+    s.table_bytes += p.nd ? p.nd->table_bytes() : p.od->table_bytes();
+  }
+  if (!lat_ns_.empty()) {
+    std::vector<int64_t> v(lat_ns_);
+    std::sort(v.begin(), v.end());
+    auto rank = [&](double q) {  // nearest-rank percentile
+      size_t k = (size_t)std::ceil(q * (double)v.size());
+      return (double)v[k ? k - 1 : 0] * 1e-6;
+    };
+    s.p50_ms = rank(0.50);
+    s.p99_ms = rank(0.99);
+    double sum = 0;
+    for (int64_t x : v) sum += (double)x;
+    s.mean_ms = sum / (double)v.size() * 1e-6;
+    s.max_ms = (double)v.back() * 1e-6;
+    s.span_s = (double)(last_done_ - first_arrival_) * 1e-9;
+  }
+  s.gflop = gflop_;
+  return s;
+}
+
+void Server::reset_stats() {
+  std::lock_guard<std::mutex> lk(mu_);
+  n_submitted_ = n_completed_ = n_failed_ = 0;
+  lat_ns_.clear();
+  first_arrival_ = INT64_MAX;
+  last_done_ = 0;
+  gflop_ = 0;
+  arena_.reset_peak();
+}
+
+}  // namespace sa

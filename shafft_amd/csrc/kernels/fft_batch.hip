@@ -91,15 +91,17 @@ extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int bloc
                      b->done, n, b->head, b->err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return (int)e;
-  static bool attr[2] = {false, false};
-  if (!attr[prec ? 1 : 0]) {
-    const void* f = prec ? reinterpret_cast<const void*>(&sa::fft_batch_kernel<double>)
-                         : reinterpret_cast<const void*>(&sa::fft_batch_kernel<float>);
-    // 160 KiB minus room for the kernel's static LDS (the queue item slot).
-    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
+  // 160 KiB minus room for the kernel's static LDS (the queue item slot);
+  // set once per precision (thread-safe static init).
+  static const bool attr = [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sa::fft_batch_kernel<float>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sa::fft_batch_kernel<double>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
     (void)hipGetLastError();
-    attr[prec ? 1 : 0] = true;
-  }
+    return true;
+  }();
+  (void)attr;
   if (prec)
     hipLaunchKernelGGL(sa::fft_batch_kernel<double>, dim3((uint32_t)grid), dim3((uint32_t)block), lds, st, *b);
   else

@@ -345,13 +345,13 @@ template <typename R, int N, int SIGN>
 hipError_t launch_n(const sa_pass_t* p, hipStream_t s, int grid) {
   const uint32_t nthr = (uint32_t)((N / Elems<R>::E) << p->logC);
   const size_t lds = ((size_t)p->ldsstride << p->logC) * sizeof(cpx<R>);
-  static bool attr = false;
-  if (!attr) {
+  static const bool attr = [] {  // once per instantiation, thread-safe
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_fixed_pass<R, N, SIGN>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipGetLastError();
-    attr = true;
-  }
+    return true;
+  }();
+  (void)attr;
   hipLaunchKernelGGL((fft_fixed_pass<R, N, SIGN>), dim3((uint32_t)grid), dim3(nthr), lds, s, *p);
   return hipGetLastError();
 }

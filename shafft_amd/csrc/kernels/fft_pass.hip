@@ -42,13 +42,13 @@ static hipError_t launch_t(const sa_pass_t* p, hipStream_t st) {
   const uint32_t nthr = (uint32_t)(p->C * p->T);
   const uint64_t nblk = ((uint64_t)p->batch + p->C - 1) / p->C;
   const size_t lds = (size_t)p->C * p->ldsstride * sizeof(cpx<R>);
-  static bool attr_done = false;
-  if (!attr_done) {
+  static const bool attr = [] {  // once per instantiation, thread-safe
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_pass<R, SIGN, MODE>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, sa_max_lds_bytes());
     (void)hipGetLastError();
-    attr_done = true;
-  }
+    return true;
+  }();
+  (void)attr;
   hipLaunchKernelGGL((fft_lds_pass<R, SIGN, MODE>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
   return hipGetLastError();
 }

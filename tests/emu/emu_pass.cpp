@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <set>
 #include <vector>
 #include <cstring>
@@ -24,9 +25,15 @@
 namespace {
 
 // Allocation registry so every emulated access can be bounds-checked.
+// Plans may be built on other host threads while a pass is replayed, so the
+// registry is guarded.
 std::map<uintptr_t, size_t>& allocs() {
   static std::map<uintptr_t, size_t> m;
   return m;
+}
+std::mutex& allocs_mu() {
+  static std::mutex mu;
+  return mu;
 }
 
 [[noreturn]] void oob(const char* what, const void* base, int64_t idx) {
@@ -39,6 +46,7 @@ std::map<uintptr_t, size_t>& allocs() {
 void chk(const char* what, const void* base, int64_t idx, size_t esz) {
   const uintptr_t a = (uintptr_t)base + (uintptr_t)(idx * (int64_t)esz);
   if (idx < 0) oob(what, base, idx);
+  std::lock_guard<std::mutex> lk(allocs_mu());
   auto& m = allocs();
   auto it = m.upper_bound(a);
   if (it == m.begin()) oob(what, base, idx);
@@ -330,10 +338,12 @@ extern "C" int sa_launch_batch(const sa_batch_t* bp, int prec, int grid, int blo
 extern "C" {
 hipError_t hipMalloc(void** p, size_t n) {
   *p = std::malloc(n ? n : 1);
+  std::lock_guard<std::mutex> lk(allocs_mu());
   if (*p) allocs()[(uintptr_t)*p] = n;
   return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 hipError_t hipFree(void* p) {
+  std::lock_guard<std::mutex> lk(allocs_mu());
   allocs().erase((uintptr_t)p);
   std::free(p);
   return hipSuccess;
@@ -390,6 +400,9 @@ hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
   return hipSuccess;
 }
 hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+// Emulated launches complete before they return, so every event is done.
+hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+hipError_t hipSetDevice(int) { return hipSuccess; }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "emulated"; }
 hipError_t hipGetDevice(int* d) {

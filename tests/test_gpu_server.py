@@ -1,0 +1,145 @@
+"""GPU tests of the transform server: every served job gives the same bits
+alone (one slot, one job at a time) and in a mix whose jobs overlap on the
+device, the same bits as the plain backend call, and the oracle's result
+within the precision's tolerance (direct DFT for the smaller jobs; round
+trip for the larger ones)."""
+import numpy as np
+import pytest
+
+import oracle
+from shafft_amd import api
+from tests.serve_stream import draw_stream
+
+pytestmark = pytest.mark.gpu
+TOL = {0: 1e-5, 1: 1e-12}
+DT = {0: np.complex64, 1: np.complex128}
+
+# (kind, dims or n, prec, direction, normalize)
+MIX = [
+    (0, (512, 512), 1, 0, False), (1, 1 << 20, 0, 1, False), (0, (64, 64, 32), 0, 0, True),
+    (1, 1000, 1, 0, False), (0, (96, 80, 70), 1, 1, False), (1, 40009, 0, 0, False),
+    (0, (2048, 1024), 0, 1, False), (1, 360, 0, 1, True), (0, (64, 64, 32), 0, 0, True),
+    (1, 1 << 21, 1, 0, False), (0, (30, 28, 25), 0, 0, False), (1, 12000, 1, 1, True),
+    (0, (4096, 256), 1, 0, False), (0, (7, 11, 13), 1, 1, False), (1, 3 * 5 * 7 * 512, 0, 0, False),
+    (0, (128, 128, 128), 0, 1, False), (1, 4096, 1, 0, False), (0, (250, 240), 0, 0, True),
+]
+ORACLE_MAX = 1 << 16
+
+
+def _shape(job):
+    return (job[1],) if job[0] == 1 else tuple(job[1])
+
+
+def _submit(srv, job, src, dst):
+    kind, d, prec, direction, norm = job
+    if kind == 1:
+        return srv.submit_1d(d, prec, direction, src, dst, normalize=norm)
+    return srv.submit_nd(d, prec, direction, src, dst, normalize=norm)
+
+
+@pytest.fixture(scope="module")
+def served(gpu_lib):
+    xs, ins, oa, om = [], [], [], []
+    for i, job in enumerate(MIX):
+        x = oracle.uniform_input(_shape(job), 46 + i, DT[job[2]])
+        b = api.DeviceBuffer(x.size, job[2])
+        b.upload(x.ravel())
+        xs.append(x)
+        ins.append(b)
+        oa.append(api.DeviceBuffer(x.size, job[2]))
+        om.append(api.DeviceBuffer(x.size, job[2]))
+    alone = api.Server(slots=1, arena_bytes=1 << 30, planners=1)
+    for job, i, o in zip(MIX, ins, oa):
+        r = alone.wait(_submit(alone, job, i.ptr, o.ptr), 60)
+        assert r is not None and r.status == 0, (job, r and r.message)
+    alone.close()
+    mixed = api.Server(slots=8, arena_bytes=1 << 30, planners=2)
+    # Build the plans first so the mix itself runs from resident plans.
+    warm = [api.DeviceBuffer(x.size, j[2]) for x, j in zip(xs, MIX)]
+    for job, i, o in zip(MIX, ins, warm):
+        _submit(mixed, job, i.ptr, o.ptr)
+    mixed.drain()
+    mixed.reset_stats()
+    ids = [_submit(mixed, job, i.ptr, o.ptr) for job, i, o in zip(MIX, ins, om)]
+    res = [mixed.wait(j, 60) for j in ids]
+    stats = mixed.stats()
+    mixed.close()
+    ya = [api.download_ptr(o.ptr, x.size, j[2]).reshape(x.shape) for o, x, j in zip(oa, xs, MIX)]
+    ym = [api.download_ptr(o.ptr, x.size, j[2]).reshape(x.shape) for o, x, j in zip(om, xs, MIX)]
+    kept = [np.array_equal(api.download_ptr(i.ptr, x.size, j[2]).reshape(x.shape), x)
+            for i, x, j in zip(ins, xs, MIX)]
+    return xs, ya, ym, res, stats, kept
+
+
+def test_alone_equals_mixed(served):
+    xs, ya, ym, res, stats, kept = served
+    for job, r in zip(MIX, res):
+        assert r is not None and r.status == 0, (job, r and r.message)
+    for job, a, m in zip(MIX, ya, ym):
+        assert np.array_equal(a, m), job
+    assert all(kept)
+    assert stats["completed"] == len(MIX) and stats["failed"] == 0
+
+
+def test_mix_overlaps_on_device(served):
+    _, _, _, res, stats, _ = served
+    ev = sorted([(r.admit, 1) for r in res] + [(r.done, -1) for r in res], key=lambda e: (e[0], e[1]))
+    cur = peak = 0
+    for _, d in ev:
+        cur += d
+        peak = max(peak, cur)
+    print(f"peak concurrent jobs {peak}; p50 {stats['p50_ms']:.3f} ms p99 {stats['p99_ms']:.3f} ms")
+    assert peak >= 2
+
+
+def test_served_vs_oracle(served):
+    xs, ya, _, _, _, _ = served
+    for x, y, job in zip(xs, ya, MIX):
+        if x.size > ORACLE_MAX:
+            continue
+        sign = -1 if job[3] == 0 else 1
+        ref = oracle.dftn(x.astype(np.complex128), list(range(x.ndim)), sign)
+        if job[4]:
+            ref = ref / np.sqrt(x.size)
+        assert oracle.check_rel_error(y, ref, TOL[job[2]]), (job, oracle.rel_l2(y, ref))
+
+
+def test_served_vs_plain_calls(served):
+    xs, ya, _, _, _, _ = served
+    for x, y, job in zip(xs, ya, MIX):
+        if job[4]:
+            continue
+        prec = job[2]
+        if job[0] == 0:
+            p = api.NDBackendPlan(list(job[1]), list(range(len(job[1]))), prec)
+            a, b = api.DeviceBuffer(x.size, prec), api.DeviceBuffer(x.size, prec)
+            a.upload(x.ravel())
+            d, _ = p.execute(a.ptr, b.ptr, job[3])
+            z = api.download_ptr(d, x.size, prec).reshape(x.shape)
+        else:
+            p = api.OneDBackendPlan(job[1], prec)
+            a, b = api.DeviceBuffer(x.size, prec), api.DeviceBuffer(x.size, prec)
+            a.upload(x.ravel())
+            p.execute(a.ptr, b.ptr, job[3])
+            z = api.download_ptr(b.ptr, x.size, prec).reshape(x.shape)
+        assert np.array_equal(z, y), job
+
+
+def test_served_round_trip(gpu_lib):
+    """Large jobs: forward then backward (normalised) through the server
+    returns the input within the precision's tolerance."""
+    srv = api.Server(slots=4, arena_bytes=4 << 30, planners=2)
+    for k, r in enumerate(draw_stream(12, seed=46, lo=20, hi=24)):
+        shape = (r.n,) if r.kind == 1 else tuple(r.dims)
+        x = oracle.uniform_input(shape, 460 + k, DT[r.prec])
+        a, b, c = (api.DeviceBuffer(x.size, r.prec) for _ in range(3))
+        a.upload(x.ravel())
+        sub = srv.submit_1d if r.kind == 1 else srv.submit_nd
+        arg = r.n if r.kind == 1 else r.dims
+        j1 = sub(arg, r.prec, 0, a.ptr, b.ptr, normalize=True)
+        assert srv.wait(j1, 60).status == 0
+        j2 = sub(arg, r.prec, 1, b.ptr, c.ptr, normalize=True)
+        assert srv.wait(j2, 60).status == 0
+        y = api.download_ptr(c.ptr, x.size, r.prec).reshape(shape)
+        assert oracle.check_rel_error(y, x, TOL[r.prec]), (r.describe(), oracle.rel_l2(y, x))
+    srv.close()
