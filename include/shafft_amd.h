@@ -97,6 +97,8 @@ int shafftAmdNDExecutePass(shafft_amd_nd_plan* plan, int pass, void* in, void* o
                            int direction);
 int shafftAmd1DNumPasses(shafft_amd_1d_plan* plan, int* npasses);
 int shafftAmd1DPassBytes(shafft_amd_1d_plan* plan, int pass, int64_t* bytes);
+int shafftAmd1DExecutePass(shafft_amd_1d_plan* plan, int pass, void* in, void* out,
+                           int direction);
 
 /* Human-readable pass list of a plan into buf (NUL-terminated, truncated). */
 int shafftAmdNDDescribe(shafft_amd_nd_plan* plan, char* buf, int buflen);

@@ -54,6 +54,7 @@ _SIGS = {
     "shafftAmdNDNumPasses": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int)]),
     "shafftAmdNDPassBytes": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64)]),
     "shafftAmdNDExecutePass": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]),
+    "shafftAmd1DExecutePass": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]),
     "shafftAmd1DNumPasses": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int)]),
     "shafftAmd1DPassBytes": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64)]),
     "shafftAmdNDDescribe": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_int]),
@@ -379,6 +380,10 @@ class OneDBackendPlan:
         v = ct.c_int64()
         check(self.L.shafftAmd1DPassBytes(self.h, i, ct.byref(v)), "shafftAmd1DPassBytes", self.L)
         return v.value
+
+    def execute_pass(self, i: int, src: int, dst: int, direction: int):
+        check(self.L.shafftAmd1DExecutePass(self.h, i, ct.c_void_p(src), ct.c_void_p(dst), direction),
+              "shafftAmd1DExecutePass", self.L)
 
     def describe(self) -> str:
         buf = ct.create_string_buffer(1 << 16)

@@ -252,6 +252,16 @@ int shafftAmdNDExecutePass(shafft_amd_nd_plan* plan, int pass, void* in, void* o
   });
 }
 
+int shafftAmd1DExecutePass(shafft_amd_1d_plan* plan, int pass, void* in, void* out,
+                           int direction) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    if (pass < 0) fail(ERR_INVALID_DIM, "pass index out of range");
+    plan->be->execute_pass((size_t)pass, in, out, direction == 0 ? -1 : +1);
+    return 0;
+  });
+}
+
 int shafftAmd1DNumPasses(shafft_amd_1d_plan* plan, int* npasses) {
   return guarded([&] {
     if (!plan || !npasses) fail(ERR_NULLPTR, "null argument");

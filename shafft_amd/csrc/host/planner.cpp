@@ -57,6 +57,20 @@ int64_t lds_cap(int prec) {
   return hw;
 }
 
+// Longest line for a pass whose elements are strided in memory.  Such a pass
+// reads (or writes) C adjacent lines per element index, so its HBM segments
+// are C * element-size bytes; with 16 elements per thread and 512 threads
+// per workgroup, C >= 16 needs n <= 512.  Longer strided lines are split
+// (four-step) rather than run with fewer, narrower lines.
+int64_t lf_cap(int prec) {
+  int64_t c = 512;
+  if (const char* e = std::getenv("SHAFFT_AMD_LF_CAP")) {
+    const long long v = std::atoll(e);
+    if (v >= 8) c = v;
+  }
+  return std::min(c, lds_cap(prec));
+}
+
 bool is_smooth7(int64_t n) {
   if (n < 1) return false;
   for (int p : {2, 3, 5, 7})
@@ -445,6 +459,33 @@ static std::vector<int64_t> canon_strides(const std::vector<Digit>& d, int64_t l
   return s;
 }
 
+// Strides of `lines` in an intermediate buffer holding `len` elements per
+// line: lines outermost in their order, except that a digit of unit stride on
+// both sides (the contiguous rows of a strided N-D axis) stays innermost, so
+// the passes on either side keep coalesced line-fast access on it.  *U is
+// the intermediate's element stride (that digit's count, or 1).
+// *ins is where a new digit of the transform dimension goes in the line
+// list (just before that digit, so it can merge with it).
+static std::vector<int64_t> inter_strides(const std::vector<Digit>& lines, int64_t len,
+                                          int64_t* U, int* ins) {
+  int u = -1;
+  for (int j = 0; j < (int)lines.size(); ++j)
+    if (lines[j].is == 1 && lines[j].os == 1 && lines[j].cnt > 1) u = j;
+  *U = u >= 0 ? lines[u].cnt : 1;
+  *ins = u >= 0 ? u : (int)lines.size();
+  std::vector<Digit> other;
+  std::vector<int> idx;
+  for (int j = 0; j < (int)lines.size(); ++j)
+    if (j != u) {
+      other.push_back(lines[j]);
+      idx.push_back(j);
+    }
+  const auto co = canon_strides(other, len * *U);
+  std::vector<int64_t> cs(lines.size(), 1);
+  for (size_t k = 0; k < idx.size(); ++k) cs[idx[k]] = co[k];
+  return cs;
+}
+
 static int64_t choose_n1(int64_t N, int64_t cap) {
   // Number of levels so that every factor fits; then the largest smooth
   // divisor not above the balanced size N^(1/levels).
@@ -473,31 +514,37 @@ static int64_t choose_n1(int64_t N, int64_t cap) {
 void PassProgram::emit(int64_t N, std::vector<Digit> lines, int64_t iks, int64_t oks,
                        int sign, const ElemHook& ld, const ElemHook& st) {
   const int64_t cap = lds_cap(prec_);
+  const int64_t lfcap = lf_split_ ? lf_cap(prec_) : cap;
   const int64_t L = prod_cnt(lines);
-  if (N <= cap && is_smooth7(N)) {
+  const bool strided = iks != 1 || oks != 1;
+  if (N <= (strided ? lfcap : cap) && is_smooth7(N)) {
     emit_single(N, lines, iks, oks, sign, ld, st, nullptr, -1, 0, L * N);
     return;
   }
   if (!is_smooth7(N)) fail(ERR_INTERNAL, "emit: non-smooth length reached four-step");
-  // Four-step N = N1 * N2.
-  const int64_t N1 = choose_n1(N, cap);
+  // Four-step N = N1 * N2; both passes have a strided side.
+  const int64_t N1 = choose_n1(N, lfcap);
   const int64_t N2 = N / N1;
   if (N1 < 2 || N2 < 2) fail(ERR_NOT_IMPL, "no four-step factorisation for this length");
 
+  // Intermediate layout [lines][k1][n2][u] (u: a unit-stride row digit, or
+  // none); U is the stride of n2 in it.
+  int64_t U = 1;
+  int ins = 0;
+  const std::vector<int64_t> cs = inter_strides(lines, N, &U, &ins);
+  auto shift = [ins](int lv) { return lv >= ins ? lv + 1 : lv; };
   // Pass A: N1-point DFTs along n1 (element stride N2*iks), one line per
-  // (lines..., n2); output in canonical [lines][k1][n2] with W_N^{n2 k1}.
+  // (lines..., n2); output k1 at stride N2*U with W_N^{n2 k1}.
   std::vector<Digit> la = lines;
-  {
-    auto cs = canon_strides(lines, N);
-    for (size_t j = 0; j < la.size(); ++j) la[j].os = cs[j];
-    la.push_back(Digit{N2, iks, 1});
-  }
-  const int n2_level = (int)la.size() - 1;
+  for (size_t j = 0; j < la.size(); ++j) la[j].os = cs[j];
+  la.insert(la.begin() + ins, Digit{N2, iks, U});
+  const int n2_level = ins;
   // Parent index m = n * kmul + (digit terms); in pass A, n = n1 * N2 + n2.
   ElemHook lda;
   if (ld.active()) {
     if (ld.level2 >= 0) fail(ERR_NOT_IMPL, "load hook deeper than two split levels");
     lda = ld;
+    if (lda.level >= 0) lda.level = shift(lda.level);
     lda.kmul = N2 * ld.kmul;
     if (ld.level >= 0) {
       lda.level2 = n2_level;
@@ -507,16 +554,13 @@ void PassProgram::emit(int64_t N, std::vector<Digit> lines, int64_t iks, int64_t
       lda.dmul = ld.kmul;
     }
   }
-  emit_single(N1, la, N2 * iks, N2, sign, lda, ElemHook{}, nullptr, n2_level, N, L * N);
+  emit_single(N1, la, N2 * iks, N2 * U, sign, lda, ElemHook{}, nullptr, n2_level, N, L * N);
 
-  // Job B: N2-point DFTs along n2 (unit stride in the intermediate), one line
+  // Pass B: N2-point DFTs along n2 (stride U in the intermediate), one line
   // per (lines..., k1); the store writes X[k1 + N1 k2] in the caller layout.
   std::vector<Digit> lb = lines;
-  {
-    auto cs = canon_strides(lines, N);
-    for (size_t j = 0; j < lb.size(); ++j) lb[j].is = cs[j];
-    lb.push_back(Digit{N1, N2, oks});
-  }
+  for (size_t j = 0; j < lb.size(); ++j) lb[j].is = cs[j];
+  lb.insert(lb.begin() + ins, Digit{N1, N2 * U, oks});
   // Parent index m = k * kmul + (digit terms) with k = k1 + N1 * k2.
   ElemHook stb;
   if (st.active()) {
@@ -524,21 +568,41 @@ void PassProgram::emit(int64_t N, std::vector<Digit> lines, int64_t iks, int64_t
     stb = st;
     stb.kmul = N1 * st.kmul;
     if (st.level >= 0) {
-      stb.level2 = (int)lb.size() - 1;
+      stb.level = shift(st.level);
+      stb.level2 = ins;
       stb.dmul2 = st.kmul;
     } else {
-      stb.level = (int)lb.size() - 1;
+      stb.level = ins;
       stb.dmul = st.kmul;
     }
   }
-  emit(N2, lb, 1, N1 * oks, sign, ElemHook{}, stb);
+  emit(N2, lb, U, N1 * oks, sign, ElemHook{}, stb);
 }
 
 void PassProgram::add_transform(int64_t N, const std::vector<Digit>& lines, int64_t iks,
                                 int64_t oks) {
+  const size_t p0 = passes_.size();
+  lf_split_ = true;
+  try {
+    add_transform_impl(N, lines, iks, oks);
+  } catch (const Error& e) {
+    if (e.status != ERR_NOT_IMPL) throw;
+    // Splitting strided lines at the coalescing cap can need more line
+    // digits than a pass has (deep Bluestein chains on strided axes): plan
+    // that transform with the LDS cap alone.  Tables already uploaded stay.
+    passes_.resize(p0);
+    lf_split_ = false;
+    add_transform_impl(N, lines, iks, oks);
+  }
+  lf_split_ = true;
+}
+
+void PassProgram::add_transform_impl(int64_t N, const std::vector<Digit>& lines, int64_t iks,
+                                     int64_t oks) {
   if (N < 1) fail(ERR_INVALID_DIM, "transform length < 1");
   if (N == 1) return;  // identity
-  const int64_t cap = lds_cap(prec_);
+  const int64_t cap =
+      (lf_split_ && (iks != 1 || oks != 1)) ? lf_cap(prec_) : lds_cap(prec_);
   const int64_t L = prod_cnt(lines);
   if (is_smooth7(N)) {
     emit(N, lines, iks, oks, 0, ElemHook{}, ElemHook{});
@@ -573,22 +637,25 @@ void PassProgram::add_transform(int64_t N, const std::vector<Digit>& lines, int6
   }
   // Long Bluestein: forward M-FFT (chirp + zero padding on its first load,
   // filter product on its last store), then backward M-FFT (output chirp and
-  // truncation on its last store).  The intermediate is canonical [lines][M].
+  // truncation on its last store).  The intermediate is [lines][M][u] as
+  // in the four-step (u: a unit-stride row digit, or none).
   std::vector<Digit> lf = lines;
-  auto cs = canon_strides(lines, M);
+  int64_t U = 1;
+  int ins = 0;
+  const std::vector<int64_t> cs = inter_strides(lines, M, &U, &ins);
   for (size_t j = 0; j < lf.size(); ++j) lf[j].os = cs[j];
   ElemHook ld;
   ld.table = tc[0];
   ld.limit = N;
   ElemHook sth;
   sth.table = th[0];
-  emit(M, lf, iks, 1, -1, ld, sth);
+  emit(M, lf, iks, U, -1, ld, sth);
   std::vector<Digit> lbk = lines;
   for (size_t j = 0; j < lbk.size(); ++j) lbk[j].is = cs[j];
   ElemHook st;
   st.table = tc[0];
   st.limit = N;
-  emit(M, lbk, 1, oks, +1, ElemHook{}, st);
+  emit(M, lbk, U, oks, +1, ElemHook{}, st);
   // Mark direction-dependent tables and intermediate sizes.
   for (size_t i = first; i < passes_.size(); ++i) {
     PassSpec& ps = passes_[i];

@@ -87,6 +87,8 @@ struct Prec {
 
 // Largest LDS-resident line for a precision (elements).
 int64_t lds_cap(int prec);
+// Longest line for a pass with strided elements (coalescing bound).
+int64_t lf_cap(int prec);
 bool is_smooth7(int64_t n);
 std::vector<int> radix_plan(int64_t n);  // radices for a 7-smooth n (product == n)
 
@@ -125,6 +127,8 @@ class PassProgram {
  private:
   int prec_;
   std::vector<PassSpec> passes_;
+  void add_transform_impl(int64_t N, const std::vector<Digit>& lines, int64_t iks, int64_t oks);
+  bool lf_split_ = true;  // split strided lines at lf_cap (else only at lds_cap)
   std::vector<DevBuf> tables_;
   std::map<std::vector<int>, std::pair<const void*, std::vector<int>>> stage_cache_;
 

@@ -114,6 +114,49 @@ for dims in [(64, 48, 30), (7, 1000), (3, 257, 2)]:
     _run_in_subprocess(code, {"SHAFFT_AMD_LDS_CAP": "32"})
 
 
+def test_nd_strided_axis_split(emu_lib):
+    """A strided axis longer than the strided-line cap (512) is split
+    (four-step, or a four-pass Bluestein chain for a prime), with the
+    unit-stride row digit kept innermost so every pass is line-fast (C
+    adjacent rows per element) on both sides."""
+    L = emu_lib
+    for dims, prec, npass in [((1200, 6), 0, 3), ((6, 700, 5), 1, 4), ((1031, 4), 1, 5), ((600, 22), 0, 3)]:
+        x = oracle.uniform_input(dims, 7, DT[prec])
+        p = api.NDBackendPlan(dims, list(range(len(dims))), prec, L=L)
+        assert p.num_passes() == npass, (dims, p.describe())
+        for line in p.describe().strip().splitlines():
+            if "iks=1 " not in line or "oks=1 " not in line:
+                assert "ld=line st=line" in line, (dims, line)
+        a, b = api.DeviceBuffer(x.size, prec, L), api.DeviceBuffer(x.size, prec, L)
+        for d, sign in ((api.FORWARD, -1), (api.BACKWARD, 1)):
+            a.upload(x.ravel())
+            dd, _ = p.execute(a.ptr, b.ptr, d)
+            y = api.download_ptr(dd, x.size, prec, L).reshape(dims)
+            ref = oracle.dftn(x.astype(np.complex128), list(range(len(dims))), sign)
+            assert oracle.check_rel_error(y, ref, TOL[prec]), (dims, d, oracle.rel_l2(y, ref))
+
+
+def test_nd_small_strided_cap(emu_lib):
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, '.')
+import oracle
+from shafft_amd import api
+L = api.bind('_build/emu/libshafft_emu.so')
+for dims, prec in [((40, 36, 20), 1), ((3, 97, 8), 1), ((2, 64, 2, 30), 0)]:
+    p = api.NDBackendPlan(dims, list(range(len(dims))), prec, L=L)
+    x = oracle.uniform_input(dims, 46, np.complex64 if prec == 0 else np.complex128)
+    a = api.DeviceBuffer(x.size, prec, L); b = api.DeviceBuffer(x.size, prec, L); a.upload(x.ravel())
+    d, w = p.execute(a.ptr, b.ptr, 1)
+    y = api.download_ptr(d, x.size, prec, L).reshape(dims)
+    e = oracle.rel_l2(y, np.fft.ifftn(x.astype(complex)) * x.size)
+    print(dims, p.num_passes(), e)
+    assert e < (1e-5 if prec == 0 else 1e-12)
+"""
+    out = _run_in_subprocess(code, {"SHAFFT_AMD_LF_CAP": "8"})
+    assert "(40, 36, 20) 5" in out and "(3, 97, 8) 3" in out
+
+
 def test_inplace_buffers(emu_lib):
     L = emu_lib
     dims = (16, 12, 10)
