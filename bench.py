@@ -267,6 +267,10 @@ def bench_serve(args, hip):
     st = srv.stats()
     srv.close()
     gpu_busy_s = sum(r.gpu_ns for r in sat_res) * 1e-9
+    # every job queued at once on a fresh server: plans built while serving
+    srv = api.Server(slots=args.slots, arena_bytes=args.arena_gb << 30, planners=2)
+    cold_sat = _lat_stats(_serve_run(srv, reqs, inp, outs, paced=False))
+    srv.close()
 
     # plain per-call path on the same jobs: plans made beforehand (not timed)
     stream = hip.stream()
@@ -294,6 +298,24 @@ def bench_serve(args, hip):
     hip.sync()
     plain = time.perf_counter() - t0
     del plans
+    # plain per-call including the plan: init, plan, execute, destroy per job
+    # (the reference call sequence for a shape not seen before)
+    hip.sync()
+    t0 = time.perf_counter()
+    for r in reqs:
+        if r.kind == 1:
+            p = api.FFT1D()
+            p.init(r.n, r.n, 0, r.prec)
+        else:
+            p = api.FFTND()
+            p.init([1] * len(r.dims), r.dims, r.prec)
+        p.plan()
+        p.set_stream(stream)
+        p.set_buffers(scratch.ptr, outs[0].ptr)
+        p.execute(api.BACKWARD if r.direction else api.FORWARD)
+        p.release()
+    hip.sync()
+    plain_planned = time.perf_counter() - t0
 
     out = {
         "metric": "served mixed-size stream throughput (GFLOP/s, 5 N log2 N per job); p50/p99 latency",
@@ -331,6 +353,9 @@ def bench_serve(args, hip):
             "plain_per_call_gflops": round(gflop / plain, 2),
             "plain_per_call_req_per_s": round(len(reqs) / plain, 1),
             "speedup_vs_plain": round(plain / sat["span_s"], 3),
+            "cold_saturated_s": round(cold_sat["span_s"], 4),
+            "plain_with_planning_s": round(plain_planned, 4),
+            "cold_speedup_vs_plain_with_planning": round(plain_planned / cold_sat["span_s"], 3),
             "plans_resident": st["plans"],
             "table_mb": round(st["table_bytes"] / 2**20, 2),
             "arena_peak_gb": round(st["arena_peak"] / 2**30, 3),
