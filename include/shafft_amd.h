@@ -121,6 +121,11 @@ int shafftAmdBatchSetStream(shafft_amd_batch_plan* plan, void* stream);
 /* Asynchronous on the plan's stream; use_graph = 0 launches directly. */
 int shafftAmdBatchExecute(shafft_amd_batch_plan* plan, int direction, int use_graph);
 int shafftAmdBatchSynchronize(shafft_amd_batch_plan* plan);
+/* Radix-8/16 stages of a complex-float batch: 0 FP32 (default), 1 bfloat16
+ * hi/lo split precision on the matrix cores (FP32 accumulation), 2 float16
+ * hi/lo split with per-butterfly power-of-two scaling.  Other radices stay
+ * FP32.  No reference counterpart (the reference computes in FP32/FP64). */
+int shafftAmdBatchSetPrecisionMode(shafft_amd_batch_plan* plan, int mode);
 /* Times the candidate tile schedules on the attached buffers (overwrites
  * out and work), keeps the fastest and writes a report into buf. */
 int shafftAmdBatchAutotune(shafft_amd_batch_plan* plan, char* buf, int buflen);

@@ -66,6 +66,7 @@ _SIGS = {
     "shafftAmdBatchSetStream": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "shafftAmdBatchExecute": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int]),
     "shafftAmdBatchSynchronize": (ct.c_int, [ct.c_void_p]),
+    "shafftAmdBatchSetPrecisionMode": (ct.c_int, [ct.c_void_p, ct.c_int]),
     "shafftAmdBatchAutotune": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_int]),
     "shafftAmdBatchInfo": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int),
                                       ct.POINTER(ct.c_int64)]),
@@ -399,6 +400,9 @@ class OneDBackendPlan:
             pass
 
 
+FP32, SPLIT_BF16, SPLIT_FP16 = 0, 1, 2
+
+
 class BatchPlan:
     """Many transforms of uneven shape in one persistent launch
     (shafftAmdBatch* in include/shafft_amd.h)."""
@@ -430,6 +434,10 @@ class BatchPlan:
 
     def synchronize(self):
         check(self.L.shafftAmdBatchSynchronize(self.h), "shafftAmdBatchSynchronize", self.L)
+
+    def set_precision_mode(self, mode: int):
+        """0 FP32, 1 bfloat16 hi/lo split (matrix cores), 2 float16 hi/lo split."""
+        check(self.L.shafftAmdBatchSetPrecisionMode(self.h, mode), "shafftAmdBatchSetPrecisionMode", self.L)
 
     def autotune(self) -> str:
         buf = ct.create_string_buffer(4096)

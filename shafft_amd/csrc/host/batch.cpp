@@ -125,7 +125,7 @@ void BatchPlan::build() {
       p.reg_first = p.reg_last = 0;
       p.nt = 0;
       p.logC = 0;
-      block_ = std::max(block_, C * T);
+      block_ = std::max(block_, (C * T + 63) / 64 * 64);  // whole waves
       lds_ = std::max(lds_, (size_t)C * stride * esz);
       tiles[i] = (uint32_t)((p.batch + C - 1) / C);
       if (j > 0) dep[i] = (int32_t)(i - 1);
@@ -210,6 +210,7 @@ void BatchPlan::launch(int dir, hipStream_t s) {
   // Two seconds of the device wall clock: far beyond any legitimate wait
   // for one pass of one transform.
   b.wait_ticks = 2ull * wall_clock_hz();
+  b.mfma = mfma_;
   b.npasses = (uint32_t)prog_.npasses();
   const sa_batch_item_t* all = static_cast<const sa_batch_item_t*>(d_items_.p);
   if (!sched_.levels) {
@@ -229,6 +230,15 @@ void BatchPlan::launch(int dir, hipStream_t s) {
     const int g = (int)std::min<int64_t>((int64_t)n, grid_);
     hip_check((hipError_t)sa_launch_batch(&b, prec_, g, block_, lds_, s), "batch launch");
   }
+}
+
+void BatchPlan::set_precision_mode(int mode) {
+  if (mode < 0 || mode > 2) fail(ERR_INVALID_DIM, "precision mode must be 0, 1 or 2");
+  if (mode != 0 && prec_ != 0)
+    fail(ERR_INVALID_FFTTYPE, "split-precision stages are for complex float (C2C) batches");
+  if (mode == mfma_) return;
+  mfma_ = mode;
+  drop_graphs();  // captured launches carry the mode
 }
 
 void BatchPlan::execute(int dir, bool use_graph) {

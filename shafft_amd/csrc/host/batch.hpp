@@ -42,6 +42,10 @@ class BatchPlan {
   void execute(int dir, bool use_graph);
   // Wait for the stream; fails if the last launch reported a timed-out wait.
   void synchronize();
+  // Radix-8/16 stages: 0 FP32 (default), 1 bfloat16 hi/lo split on the
+  // matrix cores, 2 float16 hi/lo split (complex float batches only).
+  void set_precision_mode(int mode);
+  int precision_mode() const { return mfma_; }
   // Time candidate schedules on the attached buffers and keep the fastest.
   std::string autotune();
   const BatchSchedule& schedule() const { return sched_; }
@@ -70,6 +74,7 @@ class BatchPlan {
   hipGraphExec_t graph_[2] = {nullptr, nullptr};
   hipGraph_t gdef_[2] = {nullptr, nullptr};
   bool built_ = false;
+  int mfma_ = 0;
 
   void build();  // descriptors, work list and device arrays for sched_
   void drop_graphs();

@@ -356,6 +356,14 @@ int shafftAmdBatchSynchronize(shafft_amd_batch_plan* plan) {
   });
 }
 
+int shafftAmdBatchSetPrecisionMode(shafft_amd_batch_plan* plan, int mode) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    plan->be->set_precision_mode(mode);
+    return 0;
+  });
+}
+
 int shafftAmdBatchAutotune(shafft_amd_batch_plan* plan, char* buf, int buflen) {
   return guarded([&] {
     if (!plan) fail(ERR_NULLPTR, "null plan");

@@ -29,7 +29,7 @@ __global__ void batch_clear(uint32_t* done, uint32_t n, uint32_t* head, uint32_t
   }
 }
 
-template <typename R>
+template <typename R, int MX>
 __global__ __launch_bounds__(512) void fft_batch_kernel(const sa_batch_t b) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ uint32_t s_item;
@@ -68,9 +68,9 @@ __global__ __launch_bounds__(512) void fft_batch_kernel(const sa_batch_t b) {
     if (kind == 2)
       pass_tile<R, -1, MODE_BLUE>(a, tile, lds);
     else if (kind == 0)
-      pass_tile<R, -1, MODE_LDS>(a, tile, lds);
+      pass_tile<R, -1, MODE_LDS, MX>(a, tile, lds);
     else
-      pass_tile<R, +1, MODE_LDS>(a, tile, lds);
+      pass_tile<R, +1, MODE_LDS, MX>(a, tile, lds);
     if (!b.independent) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       __syncthreads();
@@ -92,26 +92,35 @@ extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int bloc
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return (int)e;
   // 160 KiB minus room for the kernel's static LDS (the queue item slot);
-  // set once per precision (thread-safe static init).
+  // set once per variant (thread-safe static init).
   static const bool attr = [] {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sa::fft_batch_kernel<float>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sa::fft_batch_kernel<double>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
+    const void* fs[4] = {reinterpret_cast<const void*>(&sa::fft_batch_kernel<float, 0>),
+                         reinterpret_cast<const void*>(&sa::fft_batch_kernel<float, 1>),
+                         reinterpret_cast<const void*>(&sa::fft_batch_kernel<float, 2>),
+                         reinterpret_cast<const void*>(&sa::fft_batch_kernel<double, 0>)};
+    for (const void* f : fs)
+      (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
     (void)hipGetLastError();
     return true;
   }();
   (void)attr;
+  if (prec && b->mfma) return (int)hipErrorInvalidValue;  // split stages are complex-float only
+  if ((block & 63) != 0) return (int)hipErrorInvalidValue;  // matrix-core stages need whole waves
+  const dim3 g((uint32_t)grid), t((uint32_t)block);
   if (prec)
-    hipLaunchKernelGGL(sa::fft_batch_kernel<double>, dim3((uint32_t)grid), dim3((uint32_t)block), lds, st, *b);
+    hipLaunchKernelGGL((sa::fft_batch_kernel<double, 0>), g, t, lds, st, *b);
+  else if (b->mfma == 1)
+    hipLaunchKernelGGL((sa::fft_batch_kernel<float, 1>), g, t, lds, st, *b);
+  else if (b->mfma == 2)
+    hipLaunchKernelGGL((sa::fft_batch_kernel<float, 2>), g, t, lds, st, *b);
   else
-    hipLaunchKernelGGL(sa::fft_batch_kernel<float>, dim3((uint32_t)grid), dim3((uint32_t)block), lds, st, *b);
+    hipLaunchKernelGGL((sa::fft_batch_kernel<float, 0>), g, t, lds, st, *b);
   return (int)hipGetLastError();
 }
 
 extern "C" int sa_batch_occupancy(int prec, int block, size_t lds) {
-  const void* f = prec ? reinterpret_cast<const void*>(&sa::fft_batch_kernel<double>)
-                       : reinterpret_cast<const void*>(&sa::fft_batch_kernel<float>);
+  const void* f = prec ? reinterpret_cast<const void*>(&sa::fft_batch_kernel<double, 0>)
+                       : reinterpret_cast<const void*>(&sa::fft_batch_kernel<float, 0>);
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
   (void)hipGetLastError();
   int blocks = 0;

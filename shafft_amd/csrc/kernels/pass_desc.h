@@ -124,6 +124,9 @@ typedef struct {
   /* 1: the list holds one dependency level only (the launch boundary is the
    * synchronisation): no waits and no per-tile release.                   */
   uint32_t independent;
+  /* Radix-8/16 stages of complex-float passes: 0 FP32 on the vector ALUs,
+   * 1 bfloat16 hi/lo split on the matrix cores, 2 float16 hi/lo split. */
+  int32_t mfma;
 } sa_batch_t;
 
 /* Zero the queue head and the per-pass counters (a kernel, so it can sit in

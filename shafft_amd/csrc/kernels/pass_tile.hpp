@@ -98,6 +98,10 @@ __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdi
   __syncthreads();
 }
 
+}  // namespace sa
+#include "mfma_stage.hpp"
+namespace sa {
+
 // Stage 0 from registers (v[m] = element tl + T*m, n == EMAX*T) into LDS.
 template <typename R, int RAD, int SIGN>
 __device__ __forceinline__ void stage_first_regs(const cpx<R> (&v)[EMAX], cpx<R>* line, int tl,
@@ -139,9 +143,24 @@ __device__ __forceinline__ void stage_last_regs(cpx<R> (&v)[EMAX], const cpx<R>*
   }
 }
 
-template <typename R, int SIGN>
-__device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>* line, int tl) {
+// MX != 0 (complex float only): radix-8/16 stages on the matrix cores in
+// split precision (mfma_stage.hpp); `lds` is the tile's LDS base.
+template <typename R, int SIGN, int MX = 0>
+__device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>* line, int tl,
+                                               cpx<R>* lds = nullptr) {
   const cpx<R>* tws = reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[s];
+  if constexpr (MX != 0) {
+    static_assert(sizeof(R) == 4, "split-precision stages are for complex float");
+    const int rad = uni(a.radix[s]);
+    if (rad == 16) {
+      stage_mfma<16, SIGN, MX>(lds, a.C, a.ldsstride, a.n, a.ns[s], a.nsdiv[s], tws);
+      return;
+    }
+    if (rad == 8) {
+      stage_mfma<8, SIGN, MX>(lds, a.C, a.ldsstride, a.n, a.ns[s], a.nsdiv[s], tws);
+      return;
+    }
+  }
   switch (uni(a.radix[s])) {
     case 2: stage_lds<R, 2, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
     case 3: stage_lds<R, 3, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
@@ -323,7 +342,7 @@ enum PassMode { MODE_LDS = 0, MODE_REGS = 1, MODE_BLUE = 2 };
 
 // One tile (C consecutive lines) of a pass, run by the whole workgroup.
 // Threads t >= C*T of a larger workgroup idle through the barriers.
-template <typename R, int SIGN, int MODE>
+template <typename R, int SIGN, int MODE, int MX = 0>
 __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx<R>* lds) {
   constexpr bool BLUE = MODE == MODE_BLUE;
 
@@ -377,7 +396,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   if constexpr (!BLUE) {
     const int s0 = reg_first ? 1 : 0;
     const int s1 = reg_last ? uni(a.nstages) - 1 : uni(a.nstages);
-    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN>(a, s, line, (int)tl);
+    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX>(a, s, line, (int)tl, lds);
     if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl, active);
   } else {
     const int nst = uni(a.nstages);

@@ -105,3 +105,47 @@ def test_batch_full_workload_properties(gpu_lib):
         worst_p, worst_r = max(worst_p, ep), max(worst_r, er)
     print(f"4096 transforms, {tot} elements: worst Parseval {worst_p:.3e}, worst round trip {worst_r:.3e}")
     assert worst_p < 1e-5 and worst_r < 1e-5 * np.sqrt(np.log(2 ** 24))
+
+
+# Split-precision matrix-core stages: bound on the relative L2 error per
+# transform against the FP64 oracle.  bfloat16 hi/lo keeps ~16 significand
+# bits per radix-8/16 stage (rounding 2^-17 per operand; measured 1.5e-6 ..
+# 5e-6 per transform); float16 hi/lo with per-butterfly scaling keeps ~22
+# bits (measured ~1e-7, the FP32 level), under the complex64 bound of 1e-5.
+SPLIT_TOL = {api.SPLIT_BF16: 2e-5, api.SPLIT_FP16: 1e-5}
+
+
+@pytest.mark.parametrize("mode", [api.SPLIT_BF16, api.SPLIT_FP16])
+def test_batch_split_precision(gpu_lib, mode):
+    shapes = draw_shapes(40, 9, lo=16, hi=64) + [(256, 256), (16, 128, 8), (240, 48), (7, 9, 5), (128,)]
+    bp, xs, a, o, w = setup(shapes, api.C2C)
+    bp.execute(api.FORWARD)
+    bp.synchronize()
+    base = o.download()
+    bp.set_precision_mode(mode)
+    for direction, sign in ((api.FORWARD, -1), (api.BACKWARD, +1)):
+        bp.execute(direction)
+        bp.synchronize()
+        y = o.download()
+        errs = []
+        for i, (sh, x) in enumerate(zip(shapes, xs)):
+            got = y[bp.offsets[i]:bp.offsets[i] + x.size].reshape(sh)
+            ref = oracle.dftn(x.astype(np.complex128), None, sign)
+            errs.append(oracle.rel_l2(got, ref))
+        errs = np.array(errs)
+        print(f"mode {mode} dir {direction}: relL2 vs FP64 median {np.median(errs):.2e} max {errs.max():.2e}")
+        assert errs.max() < SPLIT_TOL[mode], (mode, direction, shapes[int(errs.argmax())], errs.max())
+        if direction == api.FORWARD:
+            # the matrix-core path ran: shapes with a radix-8/16 stage differ from FP32
+            assert not np.array_equal(y, base)
+    # back to FP32: the plain result again, bit for bit
+    bp.set_precision_mode(api.FP32)
+    bp.execute(api.FORWARD)
+    bp.synchronize()
+    assert np.array_equal(o.download(), base)
+
+
+def test_batch_split_precision_rejects_c128(gpu_lib):
+    bp = api.BatchPlan([(16, 16)], api.Z2Z)
+    with pytest.raises(api.ShafftError):
+        bp.set_precision_mode(api.SPLIT_BF16)
