@@ -142,24 +142,38 @@ def bench_batch(args, hip):
     wall = (time.perf_counter() - t0) / args.steps
     ms = hip.elapsed_ms(e0, e1) / args.steps
     info = bp.info()
-    # plain per-call path
-    plans = []
-    for i, sh in enumerate(shapes):
-        p = api.FFTND()
-        p.init([1] * len(sh), sh, prec)
-        p.plan()
-        p.set_stream(stream)
-        off = bp.offsets[i] * 8
-        p.set_buffers(a.ptr + off, o.ptr + off)
-        plans.append(p)
-    for p in plans[:64]:
-        p.execute(api.FORWARD)
-    hip.sync()
-    t0 = time.perf_counter()
-    for p in plans:
-        p.execute(api.FORWARD)   # host-synchronous, result swapped into data
-    hip.sync()
-    plain = time.perf_counter() - t0
+    # FP32 vs split-precision matrix-core radix-8/16 stages: time and error
+    # against an FP64 FFT (numpy, host) on every 16th transform
+    sample = list(range(0, len(shapes), 16))
+    xs_s = [x[bp.offsets[i]:bp.offsets[i] + int(np.prod(shapes[i]))].reshape(shapes[i]) for i in sample]
+    refs = [np.fft.fftn(v.astype(np.complex128)) for v in xs_s]
+    modes = {}
+    wanted = args.batch_modes.split(",")
+    for mode, name in ((api.FP32, "fp32"), (api.SPLIT_BF16, "bf16_split"), (api.SPLIT_FP16, "fp16_split")):
+        if name not in wanted:
+            continue
+        bp.set_precision_mode(mode)
+        for _ in range(args.warmup):
+            bp.execute(api.FORWARD)
+        bp.synchronize()
+        hip.record(e0, stream)
+        for _ in range(args.steps):
+            bp.execute(api.FORWARD)
+        hip.record(e1, stream)
+        bp.synchronize()
+        mms = hip.elapsed_ms(e0, e1) / args.steps
+        errs = []
+        for i, ref in zip(sample, refs):
+            y = api.download_ptr(o.ptr + bp.offsets[i] * 8, ref.size, prec).reshape(ref.shape)
+            errs.append(float(np.linalg.norm(y - ref) / np.linalg.norm(ref)))
+        modes[name] = {"gpu_ms": round(mms, 4), "gflops": round(flops / (mms * 1e-3) / 1e9, 1),
+                       "relL2_vs_fp64_median": float(f"{np.median(errs):.3e}"),
+                       "relL2_vs_fp64_max": float(f"{np.max(errs):.3e}")}
+    bp.set_precision_mode(api.FP32)
+    if args.skip_plain:
+        plain = float("nan")
+    else:
+        plain = _batch_plain(shapes, prec, bp, a, o, stream, hip)
     out = {
         "metric": "batched uneven FFTND throughput (GFLOP/s, 5 N log2 N per transform)",
         "value": round(flops / (wall * 1e-3 * 1e3) / 1e9, 2),
@@ -178,17 +192,43 @@ def bench_batch(args, hip):
                         f"lengths in [16,256], one persistent graph launch per step (forward)",
             "elements": tot,
             "gpu_ms_per_batch": round(ms, 4),
-            "plain_per_call_ms": round(plain * 1e3, 2),
-            "speedup_vs_plain": round(plain / wall, 2),
-            "plain_gflops": round(flops / plain / 1e9, 2),
+            "plain_per_call_ms": None if args.skip_plain else round(plain * 1e3, 2),
+            "speedup_vs_plain": None if args.skip_plain else round(plain / wall, 2),
+            "plain_gflops": None if args.skip_plain else round(flops / plain / 1e9, 2),
             "hbm_gbs": round(bp.info()["bytes"] / (ms * 1e-3) / 1e9, 1),
             "passes": info["passes"],
             "items": info["items"],
             "grid": info["grid"],
             "schedule": tune.strip().splitlines()[-1] if tune else "",
+            "precision_modes": modes,
+            "error_sample": f"{len(sample)} transforms (every 16th), forward, vs numpy FP64 FFT",
         },
     }
     print(json.dumps(out), flush=True)
+
+
+def _batch_plain(shapes, prec, bp, a, o, stream, hip):
+    """Plain per-call path: one FFTND plan per shape, host-synchronous
+    execute per transform, as the reference API runs."""
+    from shafft_amd import api
+
+    plans = []
+    for i, sh in enumerate(shapes):
+        p = api.FFTND()
+        p.init([1] * len(sh), sh, prec)
+        p.plan()
+        p.set_stream(stream)
+        off = bp.offsets[i] * 8
+        p.set_buffers(a.ptr + off, o.ptr + off)
+        plans.append(p)
+    for p in plans[:64]:
+        p.execute(api.FORWARD)
+    hip.sync()
+    t0 = time.perf_counter()
+    for p in plans:
+        p.execute(api.FORWARD)   # host-synchronous, result swapped into data
+    hip.sync()
+    return time.perf_counter() - t0
 
 
 def _serve_run(srv, reqs, inp, outs, paced: bool):
@@ -380,6 +420,9 @@ def main():
     ap.add_argument("--arena-gb", type=int, default=24, help="server scratch arena (GiB)")
     ap.add_argument("--count", type=int, default=4096, help="transforms in the batch workload")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batch-modes", default="fp32,bf16_split,fp16_split",
+                    help="radix-8/16 stage precisions timed in the batch workload")
+    ap.add_argument("--skip-plain", action="store_true", help="batch: skip the plain per-call comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one device-to-device copy of the data buffer first (known-size "
                          "reference kernel for HBM counter scaling under rocprofv3 --pmc)")

@@ -71,6 +71,18 @@ int64_t lf_cap(int prec) {
   return std::min(c, lds_cap(prec));
 }
 
+// A strided line of length n stays one pass when it is under lf_cap, or
+// when the fixed-length kernel runs it with 16-line tiles (1024-point
+// complex float: 32 elements per thread, 512 threads, 139 KB of LDS).
+static bool strided_single(int64_t n, int prec) {
+  if (n <= lf_cap(prec)) return true;
+  if (std::getenv("SHAFFT_AMD_LF_CAP") || n > lds_cap(prec)) return false;
+  if (!sa_fixed_supported((int)n)) return false;
+  const int64_t T = n / sa_fixed_elems((int)n, prec);
+  const int64_t stride = n + (n - 1) / 16 + 1;
+  return 16 * T <= sa_fixed_max_threads(prec) && 16 * stride * (prec ? 16 : 8) <= 160 * 1024;
+}
+
 bool is_smooth7(int64_t n) {
   if (n < 1) return false;
   for (int p : {2, 3, 5, 7})
@@ -278,10 +290,11 @@ static bool configure_fixed(sa_pass_t& p, int prec, int64_t& grid) {
       p.st_kmul != 1 || p.ld_limit < n || p.st_limit < n)
     return false;
   const int esz = prec ? 16 : 8;
-  const int T = n / sa_fixed_elems(prec);
+  const int T = n / sa_fixed_elems(n, prec);
   if (T < 1) return false;
   const bool lf = !p.ld_kfast || !p.st_kfast;
-  int C = lf ? 16 : std::max(1, 256 / T);
+  // >= 256 threads; a line-fast side needs >= 16 adjacent lines per tile
+  int C = std::max(lf ? 16 : 1, 256 / T);
   const int knob = env_int(lf ? "SHAFFT_AMD_FX_C" : "SHAFFT_AMD_FX_CK", 0);
   if (knob > 0) C = knob;
   int logC = 0;
@@ -293,7 +306,17 @@ static bool configure_fixed(sa_pass_t& p, int prec, int64_t& grid) {
   while (logC > 0 && (((int64_t)T << logC) > maxt || ((int64_t)stride << logC) * esz > budget)) --logC;
   if (((int64_t)stride << logC) * esz > budget || ((int64_t)T << logC) > maxt) return false;
   const int nthr = T << logC;
-  const size_t lds = ((size_t)stride << logC) * esz;
+  size_t lds = ((size_t)stride << logC) * esz;
+  // Stage twiddles in LDS when they fit beside the line slots.
+  int64_t tw_elems = 0;
+  for (int s = 1; s < p.nstages; ++s)
+    tw_elems = std::max<int64_t>(tw_elems, (int64_t)p.twoff[s] + (int64_t)p.ns[s] * (p.radix[s] - 1));
+  p.tw_lds = 0;
+  if (env_int("SHAFFT_AMD_TW_LDS", 1) && tw_elems > 0 &&
+      lds + (size_t)tw_elems * esz <= (size_t)budget) {
+    p.tw_lds = (int32_t)tw_elems;
+    lds += (size_t)tw_elems * esz;
+  }
   const int occ = sa_fixed_occupancy(n, prec, -1, nthr, lds);
   if (occ <= 0) return false;
   const int64_t tiles = (p.batch + (1 << logC) - 1) >> logC;
@@ -517,7 +540,7 @@ void PassProgram::emit(int64_t N, std::vector<Digit> lines, int64_t iks, int64_t
   const int64_t lfcap = lf_split_ ? lf_cap(prec_) : cap;
   const int64_t L = prod_cnt(lines);
   const bool strided = iks != 1 || oks != 1;
-  if (N <= (strided ? lfcap : cap) && is_smooth7(N)) {
+  if ((strided ? (lf_split_ ? strided_single(N, prec_) : N <= cap) : N <= cap) && is_smooth7(N)) {
     emit_single(N, lines, iks, oks, sign, ld, st, nullptr, -1, 0, L * N);
     return;
   }

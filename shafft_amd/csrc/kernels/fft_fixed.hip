@@ -6,7 +6,7 @@
 //   * every Stockham index is a compile-time shift/or, and LDS addresses
 //     are one base register plus immediate offsets;
 //   * each thread owns E elements (k = tl + T*m, T = N/E threads per line;
-//     E = 16 complex doubles or 32 complex floats); the first stage runs on
+//     E = 16, or 32 for 1024-point complex float); the first stage runs on
 //     the loaded registers and, when load and store use the same lane
 //     mapping, the last stage writes its butterfly outputs straight to HBM;
 //   * the grid is persistent: each workgroup walks tiles b, b+G, b+2G, ...
@@ -22,13 +22,14 @@ namespace fx {
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
-// Elements per thread (16) and the workgroup size bound: 512 threads for
-// complex double (256 VGPRs per lane), 1024 for complex float (128 VGPRs),
-// so a 16-line tile of a 512/1024-point line is one workgroup.
-template <typename R>
+// Elements per thread (16; 32 for 1024-point complex float, whose 16-line
+// tiles need it) and the workgroup size bound of 512
+// threads (256 VGPRs per lane); a 16-line tile of a 512-point complex double
+// or 1024-point complex float line is one workgroup.
+template <typename R, int N>
 struct Elems {
-  static constexpr int E = 16;
-  static constexpr int MAXT = sizeof(R) == 8 ? 512 : 1024;
+  static constexpr int E = (sizeof(R) == 4 && N == 1024) ? 32 : 16;
+  static constexpr int MAXT = 512;
 };
 
 // Radix sequence for N (each radix divides 16).
@@ -122,8 +123,8 @@ __device__ __forceinline__ void ident(uint32_t t, bool kfast, int logC, uint32_t
 // v[m] = element tl + T*m of line b (zeros past the last line).
 template <typename R, int N>
 __device__ __forceinline__ void load_tile(const sa_pass_t& a, uint32_t b, uint32_t tl,
-                                          cpx<R> (&v)[Elems<R>::E]) {
-  constexpr int E = Elems<R>::E, T = N / E;
+                                          cpx<R> (&v)[Elems<R, N>::E]) {
+  constexpr int E = Elems<R, N>::E, T = N / E;
   if (b < (uint32_t)a.batch) {
     const Line l = line_of(a, b);
     const cpx<R>* src = reinterpret_cast<const cpx<R>*>(a.in) + l.in + (int64_t)tl * a.iks;
@@ -152,7 +153,7 @@ struct StoreCtx {
 
 template <typename R, int N>
 __device__ __forceinline__ StoreCtx<R> store_ctx(const sa_pass_t& a, uint32_t b, uint32_t tl) {
-  constexpr int T = N / Elems<R>::E;
+  constexpr int T = N / Elems<R, N>::E;
   StoreCtx<R> c;
   c.ok = b < (uint32_t)a.batch;
   const Line l = line_of(a, c.ok ? b : 0u);
@@ -164,16 +165,21 @@ __device__ __forceinline__ StoreCtx<R> store_ctx(const sa_pass_t& a, uint32_t b,
   return c;
 }
 
-// Store one output x of slot m (element tl + T*m) with twiddle and scale.
+// Four-step twiddle W^e of a store (two-level table); (1, 0) without one.
+template <typename R>
+__device__ __forceinline__ cpx<R> ptw(const sa_pass_t& a, uint64_t e) {
+  if (a.ptw_level < 0) return {R(1), R(0)};
+  const cpx<R>* lo = reinterpret_cast<const cpx<R>*>(a.ptw_lo);
+  const cpx<R>* hi = reinterpret_cast<const cpx<R>*>(a.ptw_hi);
+  return cmul(hi[e >> a.ptw_shift], lo[e & ((1ull << a.ptw_shift) - 1)]);
+}
+
+// Store one output x of slot m (element tl + T*m) with its twiddle w (from
+// ptw(), loaded ahead of the group's stores) and the scale.
 template <typename R, int SIGN>
-__device__ __forceinline__ void put(const sa_pass_t& a, const StoreCtx<R>& c, int m, uint64_t e,
+__device__ __forceinline__ void put(const sa_pass_t& a, const StoreCtx<R>& c, int m, cpx<R> w,
                                     cpx<R> x) {
-  if (a.ptw_level >= 0) {
-    const cpx<R>* lo = reinterpret_cast<const cpx<R>*>(a.ptw_lo);
-    const cpx<R>* hi = reinterpret_cast<const cpx<R>*>(a.ptw_hi);
-    const cpx<R> w = cmul(hi[e >> a.ptw_shift], lo[e & ((1ull << a.ptw_shift) - 1)]);
-    x = cmul_dir<SIGN>(x, w);
-  }
+  if (a.ptw_level >= 0) x = cmul_dir<SIGN>(x, w);
   if (a.scale != 1.0) x = cscale(x, (R)a.scale);
   st(c.dst + m * c.step, x, a.nt != 0);
 }
@@ -185,9 +191,9 @@ __device__ __forceinline__ uint64_t emod(uint64_t e0, uint64_t es, int k, uint64
 
 // Stage 0 from registers into LDS (Ns = 1: output index jb*RAD + r).
 template <typename R, int N, int SIGN>
-__device__ __forceinline__ void stage_first(const cpx<R> (&v)[Elems<R>::E], cpx<R>* line,
+__device__ __forceinline__ void stage_first(const cpx<R> (&v)[Elems<R, N>::E], cpx<R>* line,
                                             uint32_t tl) {
-  constexpr int E = Elems<R>::E, RAD = Radices<N>::r[0];
+  constexpr int E = Elems<R, N>::E, RAD = Radices<N>::r[0];
   constexpr int T = N / E, MAXB = E / RAD;
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
@@ -205,7 +211,7 @@ __device__ __forceinline__ void stage_first(const cpx<R> (&v)[Elems<R>::E], cpx<
 template <typename R, int N, int S, int SIGN>
 __device__ __forceinline__ void stage_mid(cpx<R>* line, const cpx<R>* __restrict__ tw,
                                           uint32_t tl) {
-  constexpr int E = Elems<R>::E;
+  constexpr int E = Elems<R, N>::E;
   constexpr int RAD = StageInfo<N, S>::R, NS = StageInfo<N, S>::NS;
   constexpr int T = N / E, MAXB = E / RAD, NB = N / RAD;
   cpx<R> u[MAXB][RAD];
@@ -231,12 +237,14 @@ __device__ __forceinline__ void stage_mid(cpx<R>* line, const cpx<R>* __restrict
   }
 }
 
+// twb: the stage twiddle table, in LDS or in global memory.
 template <typename R, int N, int SIGN, int S>
-__device__ __forceinline__ void mids(cpx<R>* line, const sa_pass_t& a, uint32_t tl) {
+__device__ __forceinline__ void mids(cpx<R>* line, const sa_pass_t& a, const cpx<R>* twb,
+                                     uint32_t tl) {
   if constexpr (S < Radices<N>::n - 1) {
     __syncthreads();
-    stage_mid<R, N, S, SIGN>(line, reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[S], tl);
-    mids<R, N, SIGN, S + 1>(line, a, tl);
+    stage_mid<R, N, S, SIGN>(line, twb + a.twoff[S], tl);
+    mids<R, N, SIGN, S + 1>(line, a, twb, tl);
   }
 }
 
@@ -244,12 +252,12 @@ __device__ __forceinline__ void mids(cpx<R>* line, const sa_pass_t& a, uint32_t 
 // jb + r*Ns = tl + T*(i + r*MAXB).  STORE: write them straight to HBM;
 // otherwise park them in LDS in natural order for a re-mapped store.
 template <typename R, int N, int SIGN, bool STORE>
-__device__ __forceinline__ void stage_last(const sa_pass_t& a, cpx<R>* line, uint32_t tl,
-                                           const StoreCtx<R>& c) {
-  constexpr int E = Elems<R>::E, S = Radices<N>::n - 1;
+__device__ __forceinline__ void stage_last(const sa_pass_t& a, const cpx<R>* twb, cpx<R>* line,
+                                           uint32_t tl, const StoreCtx<R>& c) {
+  constexpr int E = Elems<R, N>::E, S = Radices<N>::n - 1;
   constexpr int RAD = StageInfo<N, S>::R;
   constexpr int T = N / E, MAXB = E / RAD, NB = N / RAD;
-  const cpx<R>* tw = reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[S];
+  const cpx<R>* tw = twb + a.twoff[S];
   if constexpr (STORE) {
     if (!c.ok) return;
     const uint64_t step_r = ((uint64_t)MAXB * c.es) % c.Nt;
@@ -263,12 +271,20 @@ __device__ __forceinline__ void stage_last(const sa_pass_t& a, cpx<R>* line, uin
 #pragma unroll
       for (int r = 1; r < RAD; ++r) u[r] = cmul_dir<SIGN>(u[r], w[r - 1]);
       dft<RAD, SIGN>(u);
+      // twiddle loads of each group of 4 ahead of its stores (one wait)
       uint64_t e = emod(c.e0, c.es, i, c.Nt);
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) {
-        put<R, SIGN>(a, c, i + r * MAXB, e, u[r]);
-        e += step_r;
-        if (e >= c.Nt) e -= c.Nt;
+      for (int r0 = 0; r0 < RAD; r0 += 4) {
+        cpx<R> wv[4];
+#pragma unroll
+        for (int q = 0; q < 4 && r0 + q < RAD; ++q) {
+          wv[q] = ptw<R>(a, e);
+          e += step_r;
+          if (e >= c.Nt) e -= c.Nt;
+        }
+#pragma unroll
+        for (int q = 0; q < 4 && r0 + q < RAD; ++q)
+          put<R, SIGN>(a, c, i + (r0 + q) * MAXB, wv[q], u[r0 + q]);
       }
     }
   } else {
@@ -291,15 +307,25 @@ __device__ __forceinline__ void stage_last(const sa_pass_t& a, cpx<R>* line, uin
   }
 }
 
-template <typename R, int N, int SIGN>
-__global__ __launch_bounds__(Elems<R>::MAXT) void fft_fixed_pass(const sa_pass_t a) {
+// TWL: the stage twiddles (a.tw_lds elements) are copied into LDS after the
+// C line slots once per workgroup, so the stages read them with LDS loads
+// and the prefetched HBM loads of the next tile stay in flight through the
+// stages (a global twiddle load would wait for them: one in-order counter).
+template <typename R, int N, int SIGN, bool TWL>
+__global__ __launch_bounds__(512) void fft_fixed_pass(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(lds_raw);
-  constexpr int E = Elems<R>::E, T = N / E;
+  constexpr int E = Elems<R, N>::E, T = N / E;
   const uint32_t t = threadIdx.x;
   const int logC = a.logC;
   const int stride = a.ldsstride;
   const uint32_t C = 1u << logC;
+  const cpx<R>* twb = reinterpret_cast<const cpx<R>*>(a.tw);
+  if constexpr (TWL) {
+    cpx<R>* ltw = lds + ((size_t)stride << logC);
+    for (int i = (int)t; i < a.tw_lds; i += (int)blockDim.x) ltw[i] = twb[i];
+    twb = ltw;  // visible to all threads after the first stage's barrier
+  }
   const bool same = a.ld_kfast == a.st_kfast;
   uint32_t c, tl, sc, stl;
   ident<T>(t, a.ld_kfast != 0, logC, c, tl);
@@ -314,25 +340,32 @@ __global__ __launch_bounds__(Elems<R>::MAXT) void fft_fixed_pass(const sa_pass_t
     stage_first<R, N, SIGN>(v, line, tl);
     const uint32_t next = tile + gridDim.x;
     if (next < tiles) load_tile<R, N>(a, (next << logC) + c, tl, v);  // prefetch
-    mids<R, N, SIGN, 1>(line, a, tl);
+    mids<R, N, SIGN, 1>(line, a, twb, tl);
     __syncthreads();
     if (same) {
       const StoreCtx<R> ctx = store_ctx<R, N>(a, (tile << logC) + c, tl);
-      stage_last<R, N, SIGN, true>(a, line, tl, ctx);
+      stage_last<R, N, SIGN, true>(a, twb, line, tl, ctx);
     } else {
       // Transposing pass: natural order through LDS, then the store mapping.
       const StoreCtx<R> dummy = {};
-      stage_last<R, N, SIGN, false>(a, line, tl, dummy);
+      stage_last<R, N, SIGN, false>(a, twb, line, tl, dummy);
       __syncthreads();
       const StoreCtx<R> ctx = store_ctx<R, N>(a, (tile << logC) + sc, stl);
       if (ctx.ok) {
         const cpx<R>* sl = lds + (size_t)sc * stride;
         uint64_t e = ctx.e0;
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-          put<R, SIGN>(a, ctx, m, e, sl[lp((int)stl + T * m)]);
-          e += ctx.es;
-          if (e >= ctx.Nt) e -= ctx.Nt;
+        for (int m0 = 0; m0 < E; m0 += 4) {
+          cpx<R> wv[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            wv[q] = ptw<R>(a, e);
+            e += ctx.es;
+            if (e >= ctx.Nt) e -= ctx.Nt;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            put<R, SIGN>(a, ctx, m0 + q, wv[q], sl[lp((int)stl + T * (m0 + q))]);
         }
       }
     }
@@ -341,19 +374,24 @@ __global__ __launch_bounds__(Elems<R>::MAXT) void fft_fixed_pass(const sa_pass_t
   }
 }
 
-template <typename R, int N, int SIGN>
-hipError_t launch_n(const sa_pass_t* p, hipStream_t s, int grid) {
-  const uint32_t nthr = (uint32_t)((N / Elems<R>::E) << p->logC);
-  const size_t lds = ((size_t)p->ldsstride << p->logC) * sizeof(cpx<R>);
+template <typename R, int N, int SIGN, bool TWL>
+hipError_t launch_nt(const sa_pass_t* p, hipStream_t s, int grid) {
+  const uint32_t nthr = (uint32_t)((N / Elems<R, N>::E) << p->logC);
+  const size_t lds = (((size_t)p->ldsstride << p->logC) + (TWL ? (size_t)p->tw_lds : 0)) * sizeof(cpx<R>);
   static const bool attr = [] {  // once per instantiation, thread-safe
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_fixed_pass<R, N, SIGN>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_fixed_pass<R, N, SIGN, TWL>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipGetLastError();
     return true;
   }();
   (void)attr;
-  hipLaunchKernelGGL((fft_fixed_pass<R, N, SIGN>), dim3((uint32_t)grid), dim3(nthr), lds, s, *p);
+  hipLaunchKernelGGL((fft_fixed_pass<R, N, SIGN, TWL>), dim3((uint32_t)grid), dim3(nthr), lds, s, *p);
   return hipGetLastError();
+}
+
+template <typename R, int N, int SIGN>
+hipError_t launch_n(const sa_pass_t* p, hipStream_t s, int grid) {
+  return p->tw_lds > 0 ? launch_nt<R, N, SIGN, true>(p, s, grid) : launch_nt<R, N, SIGN, false>(p, s, grid);
 }
 
 template <typename R, int SIGN>
@@ -380,9 +418,9 @@ extern "C" int sa_fixed_supported(int n) {
          n == 4096 || n == 8192;
 }
 
-extern "C" int sa_fixed_elems(int prec) { (void)prec; return 16; }
+extern "C" int sa_fixed_elems(int n, int prec) { return (prec == 0 && n == 1024) ? 32 : 16; }
 
-extern "C" int sa_fixed_max_threads(int prec) { return prec == 0 ? 1024 : 512; }
+extern "C" int sa_fixed_max_threads(int prec) { (void)prec; return 512; }
 
 extern "C" int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -398,12 +436,14 @@ extern "C" int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid,
 extern "C" int sa_fixed_occupancy(int n, int prec, int sign, int nthr, size_t lds) {
   int blocks = 0;
   const void* f = nullptr;
+  // Forward variant with global twiddles; the LDS-twiddle variants use the
+  // same registers (lds includes the table when it is staged).
 #define SA_FX_CASE(NN)                                                                        \
   case NN:                                                                                    \
-    f = prec == 0 ? (sign < 0 ? reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<float, NN, -1>)  \
-                              : reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<float, NN, 1>))   \
-                  : (sign < 0 ? reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<double, NN, -1>) \
-                              : reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<double, NN, 1>)); \
+    f = prec == 0 ? (sign < 0 ? reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<float, NN, -1, false>)  \
+                              : reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<float, NN, 1, false>))   \
+                  : (sign < 0 ? reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<double, NN, -1, false>) \
+                              : reinterpret_cast<const void*>(&sa::fx::fft_fixed_pass<double, NN, 1, false>)); \
     break;
   switch (n) {
     SA_FX_CASE(32)

@@ -54,6 +54,7 @@ typedef struct {
   int32_t reg_last;  /* last stage straight into the stored registers    */
   int32_t nt;        /* non-temporal global loads/stores of the data     */
   int32_t logC;      /* log2(C) for the fixed-length kernel (C a power of 2) */
+  int32_t tw_lds;    /* fixed kernel: stage-twiddle elements staged in LDS (0: global) */
   sa_fastdiv_t Tdiv, Cdiv, n1div, n2div;
 
   /* Optional post-twiddle on the store: out *= W_N^{+-(digit * k) mod N}
@@ -93,7 +94,7 @@ int sa_launch_pass(const sa_pass_t* p, int prec, int sign, void* stream);
  * store hooks other than the four-step twiddle and scale).  `grid` is the
  * number of persistent workgroups. */
 int sa_fixed_supported(int n);
-int sa_fixed_elems(int prec); /* elements per thread (N / threads per line) */
+int sa_fixed_elems(int n, int prec); /* elements per thread (n / threads per line) */
 int sa_fixed_max_threads(int prec); /* workgroup size bound */
 int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid, void* stream);
 /* Resident workgroups per CU of that kernel for a block size and LDS size. */

@@ -104,8 +104,9 @@ void check_shape(const sa_pass_t& p) {
     std::fprintf(stderr, "EMU BAD LAUNCH: %s\n", m);
     std::abort();
   };
-  const int emax = 16;
-  if (p.T < 1 || p.C < 1 || p.C * p.T > (g_fixed_launch ? 1024 : 512)) die("threads per workgroup");
+  // the fixed kernel holds 32 complex floats per thread at n = 1024
+  const int emax = g_fixed_launch && sizeof(R) == 4 && p.n == 1024 ? 32 : 16;
+  if (p.T < 1 || p.C < 1 || p.C * p.T > 512) die("threads per workgroup");
   if ((int64_t)p.n > (int64_t)emax * p.T) die("n > EMAX * T");
   for (int s = 0; s < p.nstages; ++s) {
     const int rad = p.radix[s];
@@ -226,8 +227,8 @@ extern "C" int sa_fixed_supported(int n) {
 }
 
 extern "C" int sa_fixed_occupancy(int, int, int, int, size_t) { return 1; }
-extern "C" int sa_fixed_elems(int) { return 16; }
-extern "C" int sa_fixed_max_threads(int prec) { return prec == 0 ? 1024 : 512; }
+extern "C" int sa_fixed_elems(int n, int prec) { return (prec == 0 && n == 1024) ? 32 : 16; }
+extern "C" int sa_fixed_max_threads(int) { return 512; }
 
 // The fixed kernel has the same semantics; check its own launch shape first.
 extern "C" int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid, void*) {
@@ -236,10 +237,19 @@ extern "C" int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid,
     std::abort();
   };
   if (!sa_fixed_supported(p->n)) die("length");
-  if (p->T * sa_fixed_elems(prec) != p->n) die("T");
+  if (p->T * sa_fixed_elems(p->n, prec) != p->n) die("T");
   if (p->C != (1 << p->logC)) die("C");
   if (p->C * p->T > sa_fixed_max_threads(prec)) die("threads");
-  if ((int64_t)p->C * p->ldsstride * (prec ? 16 : 8) > 160 * 1024) die("lds");
+  if (((int64_t)p->C * p->ldsstride + p->tw_lds) * (prec ? 16 : 8) > 160 * 1024) die("lds");
+  if (p->tw_lds < 0) die("tw_lds");
+  if (p->tw_lds > 0) {
+    // the staged table must cover every twiddle a stage reads
+    int64_t need = 0;
+    for (int s = 1; s < p->nstages; ++s)
+      need = std::max<int64_t>(need, (int64_t)p->twoff[s] + (int64_t)p->ns[s] * (p->radix[s] - 1));
+    if (p->tw_lds < need) die("tw_lds below the stage table");
+    chk("tw", p->tw, p->tw_lds - 1, prec ? 16 : 8);
+  }
   if (p->ldsstride < p->n + (p->n - 1) / 16) die("stride");
   if (grid < 1) die("grid");
   if (p->pre || p->post || p->blu_h || p->ld_level >= 0 || p->st_level >= 0) die("hooks");
