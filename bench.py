@@ -231,6 +231,73 @@ def _batch_plain(shapes, prec, bp, a, o, stream, hip):
     return time.perf_counter() - t0
 
 
+def bench_fft1d(args, hip):
+    """FFT1D of length n at one rank (four-step / Bluestein), forward
+    execute through the seam with the input preserved; Gaussian inputs."""
+    from shafft_amd import api
+
+    n = args.n
+    prec = api.Z2Z if args.prec == "z2z" else api.C2C
+    rng = np.random.default_rng(44)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(
+        np.complex128 if prec == api.Z2Z else np.complex64)
+    p = api.OneDBackendPlan(n, prec)
+    stream = hip.stream()
+    p.set_stream(stream)
+    npass = p.num_passes()
+    pass_bytes = [p.pass_bytes(i) for i in range(npass)]
+    # buffers cover every pass's extents (Bluestein intermediates span M)
+    ext = max(max(p.pass_extents(i)) for i in range(npass)) if npass else n
+    a, b = api.DeviceBuffer(ext, prec), api.DeviceBuffer(ext, prec)
+    a.upload(x)
+    del x
+    for _ in range(args.warmup):
+        p.execute(a.ptr, b.ptr, api.FORWARD)
+    p.synchronize()
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    for _ in range(args.steps):
+        p.execute(a.ptr, b.ptr, api.FORWARD)
+    hip.record(e1, stream)
+    p.synchronize()
+    ms = hip.elapsed_ms(e0, e1) / args.steps
+    # per pass, each launched alone between events
+    per = []
+    for i in range(npass):
+        hip.record(e0, stream)
+        for _ in range(args.steps):
+            p.execute_pass(i, a.ptr, ext, b.ptr, ext, api.FORWARD)
+        hip.record(e1, stream)
+        per.append(hip.elapsed_ms(e0, e1) / args.steps)
+    flop = 5 * n * math.log2(n)
+    out = {
+        "metric": "FFT1D GFLOP/s (5 N log2 N)",
+        "value": round(flop / (ms * 1e-3) / 1e9, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == api.Z2Z else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"FFT1D {'complex128' if prec == api.Z2Z else 'complex64'} N={n}, one rank, forward, "
+                        f"input preserved",
+            "passes": npass,
+            "plan": p.describe().strip().splitlines(),
+            "pass_ms": [round(v, 4) for v in per],
+            "pass_gbs": [round(b / (v * 1e-3) / 1e9, 1) for b, v in zip(pass_bytes, per)],
+            "hbm_frac": round(sum(pass_bytes) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "compulsory_hbm_frac": round(2 * n * (16 if prec == api.Z2Z else 8) / (ms * 1e-3) / 1e9
+                                         / HBM_PEAK_GBS, 4),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _serve_run(srv, reqs, inp, outs, paced: bool):
     """Submit `reqs` to the server: at their Poisson arrival times (paced)
     or all at once.  Output buffers are reused round-robin, so request i
@@ -412,7 +479,8 @@ def main():
     ap.add_argument("--dims", type=str, default="512,512,512")
     ap.add_argument("--prec", type=str, default="z2z", choices=["c2c", "z2z"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="fftnd", choices=["fftnd", "batch", "serve"])
+    ap.add_argument("--workload", default="fftnd", choices=["fftnd", "batch", "serve", "fft1d"])
+    ap.add_argument("--n", type=int, default=1 << 27, help="fft1d length")
     ap.add_argument("--requests", type=int, default=2000, help="jobs in the serve workload")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve arrival rate (jobs/s)")
     ap.add_argument("--slots", type=int, default=8, help="server stream slots")
@@ -437,6 +505,9 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args, Hip())
+        return
+    if args.workload == "fft1d":
+        bench_fft1d(args, Hip())
         return
 
     from shafft_amd import api

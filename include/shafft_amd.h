@@ -89,16 +89,23 @@ int shafftAmd1DNormalize(shafft_amd_1d_plan* plan, void* data, int normExponent,
                          size_t localCount);
 
 /* Introspection used by the benchmark and profiles (no reference
- * counterpart): number of LDS passes, algorithmic HBM bytes of pass i, and a
- * single-pass launch in -> out (asynchronous on the plan's stream). */
+ * counterpart): number of LDS passes, algorithmic HBM bytes of pass i, the
+ * element counts pass i reads and writes (a Bluestein chain's intermediate
+ * passes span the padded length), and a single-pass launch in -> out
+ * (asynchronous on the plan's stream) that refuses buffers smaller than
+ * those extents. */
 int shafftAmdNDNumPasses(shafft_amd_nd_plan* plan, int* npasses);
 int shafftAmdNDPassBytes(shafft_amd_nd_plan* plan, int pass, int64_t* bytes);
-int shafftAmdNDExecutePass(shafft_amd_nd_plan* plan, int pass, void* in, void* out,
-                           int direction);
+int shafftAmdNDPassExtents(shafft_amd_nd_plan* plan, int pass, int64_t* in_elems,
+                           int64_t* out_elems);
+int shafftAmdNDExecutePass(shafft_amd_nd_plan* plan, int pass, void* in, int64_t in_elems,
+                           void* out, int64_t out_elems, int direction);
 int shafftAmd1DNumPasses(shafft_amd_1d_plan* plan, int* npasses);
 int shafftAmd1DPassBytes(shafft_amd_1d_plan* plan, int pass, int64_t* bytes);
-int shafftAmd1DExecutePass(shafft_amd_1d_plan* plan, int pass, void* in, void* out,
-                           int direction);
+int shafftAmd1DPassExtents(shafft_amd_1d_plan* plan, int pass, int64_t* in_elems,
+                           int64_t* out_elems);
+int shafftAmd1DExecutePass(shafft_amd_1d_plan* plan, int pass, void* in, int64_t in_elems,
+                           void* out, int64_t out_elems, int direction);
 
 /* Human-readable pass list of a plan into buf (NUL-terminated, truncated). */
 int shafftAmdNDDescribe(shafft_amd_nd_plan* plan, char* buf, int buflen);

@@ -53,8 +53,12 @@ _SIGS = {
     "shafftAmd1DNormalize": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_size_t]),
     "shafftAmdNDNumPasses": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int)]),
     "shafftAmdNDPassBytes": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64)]),
-    "shafftAmdNDExecutePass": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]),
-    "shafftAmd1DExecutePass": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]),
+    "shafftAmdNDExecutePass": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64,
+                                          ct.c_int]),
+    "shafftAmd1DExecutePass": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64,
+                                          ct.c_int]),
+    "shafftAmdNDPassExtents": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
+    "shafftAmd1DPassExtents": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
     "shafftAmd1DNumPasses": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int)]),
     "shafftAmd1DPassBytes": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64)]),
     "shafftAmdNDDescribe": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_int]),
@@ -335,9 +339,15 @@ class NDBackendPlan:
         check(self.L.shafftAmdNDPassBytes(self.h, i, ct.byref(v)), "shafftAmdNDPassBytes", self.L)
         return v.value
 
-    def execute_pass(self, i: int, src: int, dst: int, direction: int):
-        check(self.L.shafftAmdNDExecutePass(self.h, i, ct.c_void_p(src), ct.c_void_p(dst), direction),
-              "shafftAmdNDExecutePass", self.L)
+    def pass_extents(self, i: int):
+        """(elements read, elements written) by pass i."""
+        a, b = ct.c_int64(), ct.c_int64()
+        check(self.L.shafftAmdNDPassExtents(self.h, i, ct.byref(a), ct.byref(b)), "shafftAmdNDPassExtents", self.L)
+        return a.value, b.value
+
+    def execute_pass(self, i: int, src: int, src_elems: int, dst: int, dst_elems: int, direction: int):
+        check(self.L.shafftAmdNDExecutePass(self.h, i, ct.c_void_p(src), src_elems, ct.c_void_p(dst), dst_elems,
+                                              direction), "shafftAmdNDExecutePass", self.L)
 
     def describe(self) -> str:
         buf = ct.create_string_buffer(1 << 16)
@@ -382,9 +392,15 @@ class OneDBackendPlan:
         check(self.L.shafftAmd1DPassBytes(self.h, i, ct.byref(v)), "shafftAmd1DPassBytes", self.L)
         return v.value
 
-    def execute_pass(self, i: int, src: int, dst: int, direction: int):
-        check(self.L.shafftAmd1DExecutePass(self.h, i, ct.c_void_p(src), ct.c_void_p(dst), direction),
-              "shafftAmd1DExecutePass", self.L)
+    def pass_extents(self, i: int):
+        """(elements read, elements written) by pass i."""
+        a, b = ct.c_int64(), ct.c_int64()
+        check(self.L.shafftAmd1DPassExtents(self.h, i, ct.byref(a), ct.byref(b)), "shafftAmd1DPassExtents", self.L)
+        return a.value, b.value
+
+    def execute_pass(self, i: int, src: int, src_elems: int, dst: int, dst_elems: int, direction: int):
+        check(self.L.shafftAmd1DExecutePass(self.h, i, ct.c_void_p(src), src_elems, ct.c_void_p(dst), dst_elems,
+                                              direction), "shafftAmd1DExecutePass", self.L)
 
     def describe(self) -> str:
         buf = ct.create_string_buffer(1 << 16)

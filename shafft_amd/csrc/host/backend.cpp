@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <sstream>
+#include <string>
 
 namespace sa {
 
@@ -84,8 +85,29 @@ void NDBackend::execute_preserve(const void* in, void* out, void* s1, void* s2, 
   run_preserve(prog_, map_pr_, elems_, prec_, in, out, s1, s2, dir, stream);
 }
 
-void NDBackend::execute_pass(size_t i, void* in, void* out, int dir) {
+// Single-pass launch with the caller's buffer sizes checked against the
+// pass's extents (intermediate passes of a Bluestein chain span the padded
+// length, longer than the transform).
+static void check_pass_buffers(const PassProgram& prog, size_t i, int64_t data_elems,
+                               const void* in, int64_t in_elems, const void* out,
+                               int64_t out_elems) {
+  if (i >= prog.npasses()) fail(ERR_INVALID_DIM, "pass index out of range");
+  if (!in || !out) fail(ERR_NO_BUFFER, "buffer not set");
+  if (in_elems < prog.pass_in_elems(i, data_elems) || out_elems < prog.pass_out_elems(i))
+    fail(ERR_INVALID_DIM, "pass " + std::to_string(i) + " needs buffers of " +
+                              std::to_string(prog.pass_in_elems(i, data_elems)) + " / " +
+                              std::to_string(prog.pass_out_elems(i)) + " elements");
+}
+
+void NDBackend::pass_extents(size_t i, int64_t* in_elems, int64_t* out_elems) const {
   if (i >= prog_.npasses()) fail(ERR_INVALID_DIM, "pass index out of range");
+  *in_elems = prog_.pass_in_elems(i, elems_);
+  *out_elems = prog_.pass_out_elems(i);
+}
+
+void NDBackend::execute_pass(size_t i, void* in, int64_t in_elems, void* out, int64_t out_elems,
+                             int dir) {
+  check_pass_buffers(prog_, i, elems_, in, in_elems, out, out_elems);
   PassProgram::SlotMap m;
   m.in.assign(prog_.npasses(), SLOT_DATA);
   m.out.assign(prog_.npasses(), SLOT_WORK);
@@ -129,8 +151,15 @@ void OneDBackend::execute_preserve(const void* in, void* out, void* s1, void* s2
   run_preserve(prog_, map_pr_, n_, prec_, in, out, s1, s2, dir, stream);
 }
 
-void OneDBackend::execute_pass(size_t i, void* in, void* out, int dir) {
+void OneDBackend::pass_extents(size_t i, int64_t* in_elems, int64_t* out_elems) const {
   if (i >= prog_.npasses()) fail(ERR_INVALID_DIM, "pass index out of range");
+  *in_elems = prog_.pass_in_elems(i, n_);
+  *out_elems = prog_.pass_out_elems(i);
+}
+
+void OneDBackend::execute_pass(size_t i, void* in, int64_t in_elems, void* out, int64_t out_elems,
+                               int dir) {
+  check_pass_buffers(prog_, i, n_, in, in_elems, out, out_elems);
   PassProgram::SlotMap m;
   m.in.assign(prog_.npasses(), SLOT_DATA);
   m.out.assign(prog_.npasses(), SLOT_WORK);

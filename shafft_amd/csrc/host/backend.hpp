@@ -26,8 +26,10 @@ class NDBackend {
   int prec() const { return prec_; }
   int64_t elems() const { return elems_; }
   const PassProgram& program() const { return prog_; }
-  // Profiling aid: run pass i alone from `in` to `out` (asynchronous).
-  void execute_pass(size_t i, void* in, void* out, int dir);
+  // Profiling aid: run pass i alone from `in` to `out` (asynchronous); the
+  // buffers' element counts must cover the pass's extents.
+  void execute_pass(size_t i, void* in, int64_t in_elems, void* out, int64_t out_elems, int dir);
+  void pass_extents(size_t i, int64_t* in_elems, int64_t* out_elems) const;
   // in -> out with `in` preserved, asynchronous on `stream`, using the
   // caller's scratch s1/s2 (preserve_scratch_elems(0/1) elements).  The plan
   // holds no per-call state, so calls on different streams may overlap.
@@ -61,7 +63,8 @@ class OneDBackend {
   int prec() const { return prec_; }
   int64_t n() const { return n_; }
   const PassProgram& program() const { return prog_; }
-  void execute_pass(size_t i, void* in, void* out, int dir);
+  void execute_pass(size_t i, void* in, int64_t in_elems, void* out, int64_t out_elems, int dir);
+  void pass_extents(size_t i, int64_t* in_elems, int64_t* out_elems) const;
   int64_t scratch_bytes() const { return (int64_t)(s1_.bytes + s2_.bytes); }
 
  private:

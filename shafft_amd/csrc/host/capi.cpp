@@ -242,22 +242,42 @@ int shafftAmdNDPassBytes(shafft_amd_nd_plan* plan, int pass, int64_t* bytes) {
   });
 }
 
-int shafftAmdNDExecutePass(shafft_amd_nd_plan* plan, int pass, void* in, void* out,
-                           int direction) {
+int shafftAmdNDExecutePass(shafft_amd_nd_plan* plan, int pass, void* in, int64_t in_elems,
+                           void* out, int64_t out_elems, int direction) {
   return guarded([&] {
     if (!plan) fail(ERR_NULLPTR, "null plan");
     if (pass < 0) fail(ERR_INVALID_DIM, "pass index out of range");
-    plan->be->execute_pass((size_t)pass, in, out, direction == 0 ? -1 : +1);
+    plan->be->execute_pass((size_t)pass, in, in_elems, out, out_elems, direction == 0 ? -1 : +1);
     return 0;
   });
 }
 
-int shafftAmd1DExecutePass(shafft_amd_1d_plan* plan, int pass, void* in, void* out,
-                           int direction) {
+int shafftAmdNDPassExtents(shafft_amd_nd_plan* plan, int pass, int64_t* in_elems,
+                           int64_t* out_elems) {
+  return guarded([&] {
+    if (!plan || !in_elems || !out_elems) fail(ERR_NULLPTR, "null argument");
+    if (pass < 0) fail(ERR_INVALID_DIM, "pass index out of range");
+    plan->be->pass_extents((size_t)pass, in_elems, out_elems);
+    return 0;
+  });
+}
+
+int shafftAmd1DExecutePass(shafft_amd_1d_plan* plan, int pass, void* in, int64_t in_elems,
+                           void* out, int64_t out_elems, int direction) {
   return guarded([&] {
     if (!plan) fail(ERR_NULLPTR, "null plan");
     if (pass < 0) fail(ERR_INVALID_DIM, "pass index out of range");
-    plan->be->execute_pass((size_t)pass, in, out, direction == 0 ? -1 : +1);
+    plan->be->execute_pass((size_t)pass, in, in_elems, out, out_elems, direction == 0 ? -1 : +1);
+    return 0;
+  });
+}
+
+int shafftAmd1DPassExtents(shafft_amd_1d_plan* plan, int pass, int64_t* in_elems,
+                           int64_t* out_elems) {
+  return guarded([&] {
+    if (!plan || !in_elems || !out_elems) fail(ERR_NULLPTR, "null argument");
+    if (pass < 0) fail(ERR_INVALID_DIM, "pass index out of range");
+    plan->be->pass_extents((size_t)pass, in_elems, out_elems);
     return 0;
   });
 }

@@ -121,6 +121,11 @@ class PassProgram {
   const PassSpec& pass(size_t i) const { return passes_[i]; }
   int64_t total_bytes() const;
   int64_t table_bytes() const;  // device bytes of the plan's tables
+  // Elements pass i reads / writes when the program's input has data_elems.
+  int64_t pass_in_elems(size_t i, int64_t data_elems) const {
+    return i == 0 ? data_elems : passes_[i - 1].out_elems;
+  }
+  int64_t pass_out_elems(size_t i) const { return passes_[i].out_elems; }
   std::string describe() const;
   int precision() const { return prec_; }
 

@@ -259,3 +259,27 @@ def test_c_api_roundtrip_and_normalize(emu_lib):
     p.normalize()
     y = api.download_ptr(d, 100, 1, L)
     assert oracle.rel_l2(y, np.fft.fft(x) / 10) < 1e-14
+
+
+def test_single_pass_extents(emu_lib):
+    """Single-pass launches (profiling aid) check the caller's buffer sizes:
+    a Bluestein chain's intermediate passes span the padded length M > n,
+    and undersized buffers are refused instead of overrun.  With buffers of
+    the reported extents every pass runs bounds-checked."""
+    L = emu_lib
+    n = 40009
+    p = api.OneDBackendPlan(n, 1, L=L)
+    ext = [p.pass_extents(i) for i in range(p.num_passes())]
+    assert p.num_passes() == 4 and ext[0] == (n, 131072) and ext[-1] == (131072, n)
+    big = max(max(e) for e in ext)
+    a, b = api.DeviceBuffer(big, 1, L), api.DeviceBuffer(big, 1, L)
+    a.upload(oracle.uniform_input((big,), 3))
+    for i in range(p.num_passes()):
+        p.execute_pass(i, a.ptr, big, b.ptr, big, 0)
+    small_a, small_b = api.DeviceBuffer(n, 1, L), api.DeviceBuffer(n, 1, L)
+    with pytest.raises(api.ShafftError, match="needs buffers"):
+        p.execute_pass(0, small_a.ptr, n, small_b.ptr, n, 0)
+    q = api.NDBackendPlan([8, 12], [0, 1], 0, L=L)
+    assert [q.pass_extents(i) for i in range(q.num_passes())] == [(96, 96)] * q.num_passes()
+    with pytest.raises(api.ShafftError):
+        q.execute_pass(0, small_a.ptr, 95, small_b.ptr, 96, 0)
