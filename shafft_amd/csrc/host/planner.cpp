@@ -238,7 +238,7 @@ static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
   const bool lf = ld_lf || st_lf;
   int C = std::max(1, 256 / T);
   if (lf) {
-    C = std::max(C, elem_bytes == 8 ? 16 : 8);
+    C = std::max(C, 256 / elem_bytes);  // 256-byte row segments when the threads allow
     const int knob = env_int("SHAFFT_AMD_LF_LINES", 0);
     if (knob > 0) C = knob;
     // line-fast lanes walk lines: keep C within the fast digit
@@ -293,8 +293,9 @@ static bool configure_fixed(sa_pass_t& p, int prec, int64_t& grid) {
   const int T = n / sa_fixed_elems(n, prec);
   if (T < 1) return false;
   const bool lf = !p.ld_kfast || !p.st_kfast;
-  // >= 256 threads; a line-fast side needs >= 16 adjacent lines per tile
-  int C = std::max(lf ? 16 : 1, 256 / T);
+  // >= 256 threads; a line-fast side wants 256-byte row segments (32
+  // complex float / 16 complex double adjacent lines), at least 16 lines
+  int C = std::max(lf ? 256 / esz : 1, 256 / T);
   const int knob = env_int(lf ? "SHAFFT_AMD_FX_C" : "SHAFFT_AMD_FX_CK", 0);
   if (knob > 0) C = knob;
   int logC = 0;
@@ -303,7 +304,9 @@ static bool configure_fixed(sa_pass_t& p, int prec, int64_t& grid) {
   if (stride % 2 == 0) stride += 1;
   const int budget = lds_budget_bytes();
   const int64_t maxt = sa_fixed_max_threads(prec);
-  while (logC > 0 && (((int64_t)T << logC) > maxt || ((int64_t)stride << logC) * esz > budget)) --logC;
+  while (logC > (lf ? 4 : 0) &&
+         (((int64_t)T << logC) > maxt || ((int64_t)stride << logC) * esz > budget))
+    --logC;
   if (((int64_t)stride << logC) * esz > budget || ((int64_t)T << logC) > maxt) return false;
   const int nthr = T << logC;
   size_t lds = ((size_t)stride << logC) * esz;

@@ -6,7 +6,7 @@
 //   * every Stockham index is a compile-time shift/or, and LDS addresses
 //     are one base register plus immediate offsets;
 //   * each thread owns E elements (k = tl + T*m, T = N/E threads per line;
-//     E = 16, or 32 for 1024-point complex float); the first stage runs on
+//     E = 16, or 32 for 512/1024-point complex float); the first stage runs on
 //     the loaded registers and, when load and store use the same lane
 //     mapping, the last stage writes its butterfly outputs straight to HBM;
 //   * the grid is persistent: each workgroup walks tiles b, b+G, b+2G, ...
@@ -22,13 +22,13 @@ namespace fx {
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
-// Elements per thread (16; 32 for 1024-point complex float, whose 16-line
-// tiles need it) and the workgroup size bound of 512
+// Elements per thread (16; 32 for 512/1024-point complex float, whose 32-
+// and 16-line tiles need it) and the workgroup size bound of 512
 // threads (256 VGPRs per lane); a 16-line tile of a 512-point complex double
 // or 1024-point complex float line is one workgroup.
 template <typename R, int N>
 struct Elems {
-  static constexpr int E = (sizeof(R) == 4 && N == 1024) ? 32 : 16;
+  static constexpr int E = (sizeof(R) == 4 && (N == 512 || N == 1024)) ? 32 : 16;
   static constexpr int MAXT = 512;
 };
 
@@ -142,16 +142,30 @@ __device__ __forceinline__ void load_tile(const sa_pass_t& a, uint32_t b, uint32
 }
 
 // Per-line store context: destination, element step and the four-step
-// twiddle exponent e(m) = dig * (tl + T*m) mod Nt.
+// twiddle of slot m (element tl + T*m), W_Nt^{e(m)} with
+// e(m) = dig * (tl + T*m) mod Nt.  The twiddles of a line are a geometric
+// sequence, so each line looks up W^{e(0)} and the steps W^{es},
+// W^{MAXB*es} once in the two-level table and forms the rest by FP64
+// complex products (at most 24 deep: ~3e-15 relative), instead of two
+// scattered table loads per element.
 template <typename R>
 struct StoreCtx {
   cpx<R>* dst;
   int64_t step;
-  uint64_t e0, es, Nt;
+  cpx<double> w0, s1, sb;  // W^{e(0)}, W^{es}, W^{MAXB*es}
   bool ok;
 };
 
-template <typename R, int N>
+// W^e from the two-level table, widened to FP64.
+template <typename R>
+__device__ __forceinline__ cpx<double> ptw64(const sa_pass_t& a, uint64_t e) {
+  const cpx<R>* lo = reinterpret_cast<const cpx<R>*>(a.ptw_lo);
+  const cpx<R>* hi = reinterpret_cast<const cpx<R>*>(a.ptw_hi);
+  const cpx<R> h = hi[e >> a.ptw_shift], l = lo[e & ((1ull << a.ptw_shift) - 1)];
+  return cmul(cpx<double>{(double)h.x, (double)h.y}, cpx<double>{(double)l.x, (double)l.y});
+}
+
+template <typename R, int N, int MAXB>
 __device__ __forceinline__ StoreCtx<R> store_ctx(const sa_pass_t& a, uint32_t b, uint32_t tl) {
   constexpr int T = N / Elems<R, N>::E;
   StoreCtx<R> c;
@@ -159,34 +173,24 @@ __device__ __forceinline__ StoreCtx<R> store_ctx(const sa_pass_t& a, uint32_t b,
   const Line l = line_of(a, c.ok ? b : 0u);
   c.dst = reinterpret_cast<cpx<R>*>(a.out) + l.out + (int64_t)tl * a.oks;
   c.step = (int64_t)T * a.oks;
-  c.Nt = a.ptw_level >= 0 ? (uint64_t)a.ptw_N : 1;
-  c.e0 = a.ptw_level >= 0 ? (l.dig * tl) % c.Nt : 0;
-  c.es = a.ptw_level >= 0 ? (l.dig * T) % c.Nt : 0;
+  c.w0 = c.s1 = c.sb = {1.0, 0.0};
+  if (a.ptw_level >= 0) {
+    const uint64_t Nt = (uint64_t)a.ptw_N;
+    const uint64_t es = (l.dig * T) % Nt;
+    c.w0 = ptw64<R>(a, (l.dig * tl) % Nt);
+    c.s1 = ptw64<R>(a, es);
+    c.sb = ptw64<R>(a, (es * MAXB) % Nt);
+  }
   return c;
 }
 
-// Four-step twiddle W^e of a store (two-level table); (1, 0) without one.
-template <typename R>
-__device__ __forceinline__ cpx<R> ptw(const sa_pass_t& a, uint64_t e) {
-  if (a.ptw_level < 0) return {R(1), R(0)};
-  const cpx<R>* lo = reinterpret_cast<const cpx<R>*>(a.ptw_lo);
-  const cpx<R>* hi = reinterpret_cast<const cpx<R>*>(a.ptw_hi);
-  return cmul(hi[e >> a.ptw_shift], lo[e & ((1ull << a.ptw_shift) - 1)]);
-}
-
-// Store one output x of slot m (element tl + T*m) with its twiddle w (from
-// ptw(), loaded ahead of the group's stores) and the scale.
+// Store one output x of slot m with its twiddle w and the scale.
 template <typename R, int SIGN>
-__device__ __forceinline__ void put(const sa_pass_t& a, const StoreCtx<R>& c, int m, cpx<R> w,
+__device__ __forceinline__ void put(const sa_pass_t& a, const StoreCtx<R>& c, int m, cpx<double> w,
                                     cpx<R> x) {
-  if (a.ptw_level >= 0) x = cmul_dir<SIGN>(x, w);
+  if (a.ptw_level >= 0) x = cmul_dir<SIGN>(x, cpx<R>{(R)w.x, (R)w.y});
   if (a.scale != 1.0) x = cscale(x, (R)a.scale);
   st(c.dst + m * c.step, x, a.nt != 0);
-}
-
-// e0 + k*es mod Nt for small k (k*es < 2^63 guaranteed by Nt < 2^40).
-__device__ __forceinline__ uint64_t emod(uint64_t e0, uint64_t es, int k, uint64_t Nt) {
-  return (e0 + (uint64_t)k * es) % Nt;
 }
 
 // Stage 0 from registers into LDS (Ns = 1: output index jb*RAD + r).
@@ -260,7 +264,7 @@ __device__ __forceinline__ void stage_last(const sa_pass_t& a, const cpx<R>* twb
   const cpx<R>* tw = twb + a.twoff[S];
   if constexpr (STORE) {
     if (!c.ok) return;
-    const uint64_t step_r = ((uint64_t)MAXB * c.es) % c.Nt;
+    cpx<double> wi = c.w0;  // twiddle of slot i (r = 0)
 #pragma unroll
     for (int i = 0; i < MAXB; ++i) {
       const int jb = (int)tl + i * T;
@@ -271,21 +275,13 @@ __device__ __forceinline__ void stage_last(const sa_pass_t& a, const cpx<R>* twb
 #pragma unroll
       for (int r = 1; r < RAD; ++r) u[r] = cmul_dir<SIGN>(u[r], w[r - 1]);
       dft<RAD, SIGN>(u);
-      // twiddle loads of each group of 4 ahead of its stores (one wait)
-      uint64_t e = emod(c.e0, c.es, i, c.Nt);
+      cpx<double> wr = wi;  // slot i + r*MAXB
 #pragma unroll
-      for (int r0 = 0; r0 < RAD; r0 += 4) {
-        cpx<R> wv[4];
-#pragma unroll
-        for (int q = 0; q < 4 && r0 + q < RAD; ++q) {
-          wv[q] = ptw<R>(a, e);
-          e += step_r;
-          if (e >= c.Nt) e -= c.Nt;
-        }
-#pragma unroll
-        for (int q = 0; q < 4 && r0 + q < RAD; ++q)
-          put<R, SIGN>(a, c, i + (r0 + q) * MAXB, wv[q], u[r0 + q]);
+      for (int r = 0; r < RAD; ++r) {
+        put<R, SIGN>(a, c, i + r * MAXB, wr, u[r]);
+        if (r + 1 < RAD) wr = cmul(wr, c.sb);
       }
+      if (i + 1 < MAXB) wi = cmul(wi, c.s1);
     }
   } else {
     cpx<R> u[MAXB][RAD];
@@ -343,29 +339,22 @@ __global__ __launch_bounds__(512) void fft_fixed_pass(const sa_pass_t a) {
     mids<R, N, SIGN, 1>(line, a, twb, tl);
     __syncthreads();
     if (same) {
-      const StoreCtx<R> ctx = store_ctx<R, N>(a, (tile << logC) + c, tl);
+      const StoreCtx<R> ctx =
+          store_ctx<R, N, E / StageInfo<N, Radices<N>::n - 1>::R>(a, (tile << logC) + c, tl);
       stage_last<R, N, SIGN, true>(a, twb, line, tl, ctx);
     } else {
       // Transposing pass: natural order through LDS, then the store mapping.
       const StoreCtx<R> dummy = {};
       stage_last<R, N, SIGN, false>(a, twb, line, tl, dummy);
       __syncthreads();
-      const StoreCtx<R> ctx = store_ctx<R, N>(a, (tile << logC) + sc, stl);
+      const StoreCtx<R> ctx = store_ctx<R, N, 1>(a, (tile << logC) + sc, stl);
       if (ctx.ok) {
         const cpx<R>* sl = lds + (size_t)sc * stride;
-        uint64_t e = ctx.e0;
+        cpx<double> w = ctx.w0;
 #pragma unroll
-        for (int m0 = 0; m0 < E; m0 += 4) {
-          cpx<R> wv[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            wv[q] = ptw<R>(a, e);
-            e += ctx.es;
-            if (e >= ctx.Nt) e -= ctx.Nt;
-          }
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            put<R, SIGN>(a, ctx, m0 + q, wv[q], sl[lp((int)stl + T * (m0 + q))]);
+        for (int m = 0; m < E; ++m) {
+          put<R, SIGN>(a, ctx, m, w, sl[lp((int)stl + T * m)]);
+          if (m + 1 < E) w = cmul(w, ctx.s1);
         }
       }
     }
@@ -418,7 +407,7 @@ extern "C" int sa_fixed_supported(int n) {
          n == 4096 || n == 8192;
 }
 
-extern "C" int sa_fixed_elems(int n, int prec) { return (prec == 0 && n == 1024) ? 32 : 16; }
+extern "C" int sa_fixed_elems(int n, int prec) { return (prec == 0 && (n == 512 || n == 1024)) ? 32 : 16; }
 
 extern "C" int sa_fixed_max_threads(int prec) { (void)prec; return 512; }
 

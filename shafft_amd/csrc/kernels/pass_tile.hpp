@@ -288,25 +288,30 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
   const bool do_scale = a.scale != 1.0;
   const R sc = (R)a.scale;
   if (a.ptw_level >= 0) {
-    // W_N^{(digit * k) mod N}, walked incrementally along k.
+    // W_N^{(digit * k) mod N} for k = k0 + i*T is geometric in i: look up
+    // W^{e(k0)} and the step W^{digit*T} once (two-level table) and form the
+    // rest by FP64 complex products (<= 15 deep), not two scattered table
+    // loads per element.
     const uint64_t N = (uint64_t)a.ptw_N;
     const uint64_t dig = o.ptw_dig;
-    uint64_t e = (dig * k0) % N;
-    const uint64_t step = (dig * T) % N;
     const cpx<R>* __restrict__ lo = reinterpret_cast<const cpx<R>*>(a.ptw_lo);
     const cpx<R>* __restrict__ hi = reinterpret_cast<const cpx<R>*>(a.ptw_hi);
     const uint64_t mask = (1ull << a.ptw_shift) - 1;
+    auto w64 = [&](uint64_t e) {
+      const cpx<R> h = hi[e >> a.ptw_shift], l = lo[e & mask];
+      return cmul(cpx<double>{(double)h.x, (double)h.y}, cpx<double>{(double)l.x, (double)l.y});
+    };
+    cpx<double> w = w64((dig * k0) % N);
+    const cpx<double> step = w64((dig * T) % N);
 #pragma unroll
     for (int i = 0; i < EMAX; ++i) {
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
-        const cpx<R> w = cmul(hi[e >> a.ptw_shift], lo[e & mask]);
-        cpx<R> x = cmul_dir<SIGN>(v[i], w);
+        cpx<R> x = cmul_dir<SIGN>(v[i], cpx<R>{(R)w.x, (R)w.y});
         if (do_scale) x = cscale(x, sc);
         gstore(dst + (int64_t)k * a.oks, x, nt);
       }
-      e += step;
-      if (e >= N) e -= N;
+      if (i + 1 < EMAX) w = cmul(w, step);
     }
   } else if (post == nullptr && a.st_level < 0 && a.st_kmul == 1 && a.st_limit >= n) {
 #pragma unroll

@@ -104,8 +104,8 @@ void check_shape(const sa_pass_t& p) {
     std::fprintf(stderr, "EMU BAD LAUNCH: %s\n", m);
     std::abort();
   };
-  // the fixed kernel holds 32 complex floats per thread at n = 1024
-  const int emax = g_fixed_launch && sizeof(R) == 4 && p.n == 1024 ? 32 : 16;
+  // the fixed kernel holds 32 complex floats per thread at n = 512, 1024
+  const int emax = g_fixed_launch && sizeof(R) == 4 && (p.n == 512 || p.n == 1024) ? 32 : 16;
   if (p.T < 1 || p.C < 1 || p.C * p.T > 512) die("threads per workgroup");
   if ((int64_t)p.n > (int64_t)emax * p.T) die("n > EMAX * T");
   for (int s = 0; s < p.nstages; ++s) {
@@ -227,7 +227,7 @@ extern "C" int sa_fixed_supported(int n) {
 }
 
 extern "C" int sa_fixed_occupancy(int, int, int, int, size_t) { return 1; }
-extern "C" int sa_fixed_elems(int n, int prec) { return (prec == 0 && n == 1024) ? 32 : 16; }
+extern "C" int sa_fixed_elems(int n, int prec) { return (prec == 0 && (n == 512 || n == 1024)) ? 32 : 16; }
 extern "C" int sa_fixed_max_threads(int) { return 512; }
 
 // The fixed kernel has the same semantics; check its own launch shape first.
