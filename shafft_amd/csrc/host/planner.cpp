@@ -90,12 +90,19 @@ bool is_smooth7(int64_t n) {
   return n == 1;
 }
 
-std::vector<int> radix_plan(int64_t n) {
+std::vector<int> radix_plan(int64_t n, bool max8) {
   std::vector<int> r;
   int a = 0;
   while (n % 2 == 0) {
     n /= 2;
     ++a;
+  }
+  if (max8) {
+    // Radix-8 stages, the remainder as one radix 4 or 2 stage.
+    for (; a >= 3; a -= 3) r.push_back(8);
+    if (a == 2) r.push_back(4);
+    if (a == 1) r.push_back(2);
+    a = 0;
   }
   // Powers of two: radix-16 stages, the remainder as one radix 8/4/2 stage;
   // 2^(4q+1) is taken as 16^(q-1) * 8 * 4 rather than 16^q * 2.
@@ -409,7 +416,10 @@ void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, 
   // The fixed-length kernel (when eligible) dictates the radix sequence.
   const bool fixed_candidate = env_int("SHAFFT_AMD_FIXED", 1) && sa_fixed_supported((int)N) &&
                                blu_h == nullptr && !ld.active() && !st.active();
-  std::vector<int> rad = N > 1 ? (fixed_candidate ? fixed_radix_plan(N) : radix_plan(N))
+  // The per-pass complex-float kernel runs radix 8 at most (its register
+  // cap); the batch kernel and complex double keep radix 16.
+  std::vector<int> rad = N > 1 ? (fixed_candidate ? fixed_radix_plan(N)
+                                                  : radix_plan(N, standalone_ && prec_ == 0))
                                : std::vector<int>{};
   if (rad.size() > SA_MAX_STAGES) fail(ERR_INTERNAL, "too many stages");
   std::vector<int> offs;
@@ -786,7 +796,8 @@ std::string PassProgram::describe() const {
   for (size_t i = 0; i < passes_.size(); ++i) {
     const sa_pass_t& p = passes_[i].p;
     os << "pass " << i << ": n=" << p.n << " batch=" << p.batch
-       << (passes_[i].fixed ? " fixed(grid=" + std::to_string(passes_[i].grid) + ")" : "") << " radices=";
+       << (passes_[i].fixed ? " fixed(grid=" + std::to_string(passes_[i].grid) + ")" : "")
+       << " radices=";
     for (int s = 0; s < p.nstages; ++s) os << (s ? "x" : "") << p.radix[s];
     os << " iks=" << p.iks << " oks=" << p.oks << " C=" << p.C << " T=" << p.T
        << " ld=" << (p.ld_kfast ? "k" : "line") << " st=" << (p.st_kfast ? "k" : "line")

@@ -20,11 +20,48 @@
 
 namespace sa {
 
-template <typename R, int SIGN, int MODE>
-__global__ __launch_bounds__(512) void fft_lds_pass(const sa_pass_t a) {
+// Register budget.  The complex-float kernel runs plans of radix 8 at most
+// (the radix-16 butterfly sets the whole kernel's register footprint) and is
+// capped at 128 VGPRs (4 waves per SIMD): two workgroups then share a CU and
+// one's HBM loads overlap the other's LDS stages; the few spills this costs
+// are cheaper (2.08 vs 1.90 TB/s over the serving stream's 30 largest
+// complex64 jobs).  The complex-double kernel spills too much under that cap
+// (1.96 vs 3.04 TB/s) and keeps 256 VGPRs and radix 16.
+// (The cap applies to plain LDS-mode passes without element hooks; the
+// register-stage and Bluestein modes and hooked passes hold more per thread
+// and run uncapped.)
+template <int SIGN>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void fft_lds_pass_fc(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<R, SIGN, MODE>(a, blockIdx.x, reinterpret_cast<cpx<R>*>(lds_raw));
+  pass_tile<float, SIGN, MODE_LDS, 0, false>(a, blockIdx.x, reinterpret_cast<cpx<float>*>(lds_raw));
 }
+
+template <int SIGN, int MODE>
+__global__ __launch_bounds__(512) void fft_lds_pass_f(const sa_pass_t a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  pass_tile<float, SIGN, MODE, 0, false>(a, blockIdx.x, reinterpret_cast<cpx<float>*>(lds_raw));
+}
+
+template <int SIGN, int MODE>
+__global__ __launch_bounds__(512) void fft_lds_pass_d(const sa_pass_t a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  pass_tile<double, SIGN, MODE, 0, true>(a, blockIdx.x, reinterpret_cast<cpx<double>*>(lds_raw));
+}
+
+template <typename R, int SIGN, int MODE, bool CAP>
+struct PassKernel;
+template <int SIGN, int MODE, bool CAP>
+struct PassKernel<float, SIGN, MODE, CAP> {
+  static const void* fn() {
+    if constexpr (MODE == MODE_LDS && CAP) return reinterpret_cast<const void*>(&fft_lds_pass_fc<SIGN>);
+    return reinterpret_cast<const void*>(&fft_lds_pass_f<SIGN, MODE>);
+  }
+};
+template <int SIGN, int MODE, bool CAP>
+struct PassKernel<double, SIGN, MODE, CAP> {
+  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_d<SIGN, MODE>); }
+};
 
 template <typename R>
 __global__ __launch_bounds__(256) void scale_kernel(cpx<R>* __restrict__ x, R f, uint64_t count) {
@@ -37,19 +74,24 @@ __global__ __launch_bounds__(256) void scale_kernel(cpx<R>* __restrict__ x, R f,
   }
 }
 
-template <typename R, int SIGN, int MODE>
+template <typename R, int SIGN, int MODE, bool CAP = (MODE == MODE_LDS)>
 static hipError_t launch_t(const sa_pass_t* p, hipStream_t st) {
   const uint32_t nthr = (uint32_t)(p->C * p->T);
   const uint64_t nblk = ((uint64_t)p->batch + p->C - 1) / p->C;
   const size_t lds = (size_t)p->C * p->ldsstride * sizeof(cpx<R>);
   static const bool attr = [] {  // once per instantiation, thread-safe
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_pass<R, SIGN, MODE>),
+    (void)hipFuncSetAttribute(PassKernel<R, SIGN, MODE, CAP>::fn(),
                               hipFuncAttributeMaxDynamicSharedMemorySize, sa_max_lds_bytes());
     (void)hipGetLastError();
     return true;
   }();
   (void)attr;
-  hipLaunchKernelGGL((fft_lds_pass<R, SIGN, MODE>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
+  if constexpr (sizeof(R) == 4 && MODE == MODE_LDS && CAP)
+    hipLaunchKernelGGL((fft_lds_pass_fc<SIGN>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
+  else if constexpr (sizeof(R) == 4)
+    hipLaunchKernelGGL((fft_lds_pass_f<SIGN, MODE>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
+  else
+    hipLaunchKernelGGL((fft_lds_pass_d<SIGN, MODE>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
   return hipGetLastError();
 }
 
@@ -57,6 +99,10 @@ template <typename R>
 static hipError_t launch_prec(const sa_pass_t* p, int sign, hipStream_t st) {
   if (p->blu_h) return launch_t<R, -1, MODE_BLUE>(p, st);
   const bool regs = p->reg_first && p->reg_last;
+  const bool hooks = p->pre || p->post || p->ld_level >= 0 || p->st_level >= 0 ||
+                     p->ld_kmul != 1 || p->st_kmul != 1 || p->ld_limit < p->n || p->st_limit < p->n;
+  if (hooks && !regs)
+    return sign < 0 ? launch_t<R, -1, MODE_LDS, false>(p, st) : launch_t<R, +1, MODE_LDS, false>(p, st);
   if (sign < 0) return regs ? launch_t<R, -1, MODE_REGS>(p, st) : launch_t<R, -1, MODE_LDS>(p, st);
   return regs ? launch_t<R, +1, MODE_REGS>(p, st) : launch_t<R, +1, MODE_LDS>(p, st);
 }

@@ -143,12 +143,36 @@ __device__ __forceinline__ void stage_last_regs(cpx<R> (&v)[EMAX], const cpx<R>*
   }
 }
 
+// One Stockham stage of radix `rad` (wave-uniform) on this thread's line.
+// R16 = false leaves out the radix-16 butterfly (the largest register
+// footprint of all stages) for kernels whose plans use radix 8 at most.
+template <typename R, int SIGN, bool R16 = true>
+__device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t nsd,
+                                          const cpx<R>* tws, cpx<R>* line, int tl, int T) {
+  if constexpr (!R16) {
+    if (rad == 16) __builtin_trap();  // the planner never emits it here
+  }
+  switch (rad) {
+    case 2: stage_lds<R, 2, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 3: stage_lds<R, 3, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 4: stage_lds<R, 4, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 5: stage_lds<R, 5, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 7: stage_lds<R, 7, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 8: stage_lds<R, 8, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 16:
+      if constexpr (R16) stage_lds<R, 16, SIGN>(line, n, Ns, nsd, tws, tl, T);
+      break;
+    default: break;
+  }
+}
+
 // MX != 0 (complex float only): radix-8/16 stages on the matrix cores in
 // split precision (mfma_stage.hpp); `lds` is the tile's LDS base.
-template <typename R, int SIGN, int MX = 0>
+// twb: the stage twiddle table (global, or staged in LDS by the caller).
+template <typename R, int SIGN, int MX = 0, bool R16 = true>
 __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>* line, int tl,
-                                               cpx<R>* lds = nullptr) {
-  const cpx<R>* tws = reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[s];
+                                               cpx<R>* lds, const cpx<R>* twb) {
+  const cpx<R>* tws = twb + a.twoff[s];
   if constexpr (MX != 0) {
     static_assert(sizeof(R) == 4, "split-precision stages are for complex float");
     const int rad = uni(a.radix[s]);
@@ -161,16 +185,7 @@ __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>
       return;
     }
   }
-  switch (uni(a.radix[s])) {
-    case 2: stage_lds<R, 2, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 3: stage_lds<R, 3, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 4: stage_lds<R, 4, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 5: stage_lds<R, 5, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 7: stage_lds<R, 7, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 8: stage_lds<R, 8, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    case 16: stage_lds<R, 16, SIGN>(line, a.n, a.ns[s], a.nsdiv[s], tws, tl, a.T); break;
-    default: break;
-  }
+  stage_run<R, SIGN, R16>(uni(a.radix[s]), a.n, a.ns[s], a.nsdiv[s], tws, line, tl, a.T);
 }
 
 template <typename R, int SIGN>
@@ -347,7 +362,7 @@ enum PassMode { MODE_LDS = 0, MODE_REGS = 1, MODE_BLUE = 2 };
 
 // One tile (C consecutive lines) of a pass, run by the whole workgroup.
 // Threads t >= C*T of a larger workgroup idle through the barriers.
-template <typename R, int SIGN, int MODE, int MX = 0>
+template <typename R, int SIGN, int MODE, int MX = 0, bool R16 = true>
 __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx<R>* lds) {
   constexpr bool BLUE = MODE == MODE_BLUE;
 
@@ -401,15 +416,17 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   if constexpr (!BLUE) {
     const int s0 = reg_first ? 1 : 0;
     const int s1 = reg_last ? uni(a.nstages) - 1 : uni(a.nstages);
-    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX>(a, s, line, (int)tl, lds);
+    const cpx<R>* twb = reinterpret_cast<const cpx<R>*>(a.tw);
+    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX, R16>(a, s, line, (int)tl, lds, twb);
     if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl, active);
   } else {
     const int nst = uni(a.nstages);
-    for (int s = 0; s < nst; ++s) stage_dispatch<R, -1>(a, s, line, (int)tl);
+    const cpx<R>* twb = reinterpret_cast<const cpx<R>*>(a.tw);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0, R16>(a, s, line, (int)tl, lds, twb);
     const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
     for (int k = (int)tl; k < n; k += a.T) line[lp(k)] = cmul(line[lp(k)], h[k]);
     __syncthreads();
-    for (int s = 0; s < nst; ++s) stage_dispatch<R, +1>(a, s, line, (int)tl);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, +1, 0, R16>(a, s, line, (int)tl, lds, twb);
   }
 
   // ---- store -----------------------------------------------------------------
