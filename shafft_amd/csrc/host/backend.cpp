@@ -1,6 +1,7 @@
 #include "backend.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <sstream>
 #include <string>
 
@@ -12,6 +13,62 @@ static void ensure_scratch(const PassProgram::SlotMap& m, int prec, DevBuf& s1, 
   const size_t esz = prec ? 16 : 8;
   if ((size_t)m.scratch[0] * esz > s1.bytes) s1 = DevBuf((size_t)m.scratch[0] * esz);
   if ((size_t)m.scratch[1] * esz > s2.bytes) s2 = DevBuf((size_t)m.scratch[1] * esz);
+}
+
+// SHAFFT_AMD_ROTATE: 0 never, 1 always, unset: when the axis-by-axis plan's
+// outermost pass would read and write complex64 rows 4 MiB or more apart.
+// Its tiles are 16 rows wide there (128-byte segments; 32 rows of 1024
+// points do not fit LDS), and on MI355X such a pass runs at ~3 TB/s (1024^3:
+// 3.06 TB/s; a plain copy with the same tiles 3.3 TB/s) while the rotated
+// plan's strided side is one axis length; measured 1024^3 complex64:
+// 12.04 -> 10.72 ms.  Complex double tiles are 256 bytes wide and the
+// axis-by-axis plan is faster there (512^3: 2.52 vs 2.71 ms).
+int NDBackend::rotate_mode() {
+  const char* e = std::getenv("SHAFFT_AMD_ROTATE");
+  return (e && *e) ? std::atoi(e) : -1;
+}
+
+// Rotated plan for a block of transformed axes that ends at the last (unit
+// stride) axis.  The block's layout is rotated left by one axis per pass:
+// with block axes a1..am and layout L0 = [a1..am], pass t transforms the
+// innermost axis of L(t-1) and writes L(t) = rotl(L(t-1)), in which that
+// axis is second innermost and the next axis to transform is innermost.
+// After m passes the layout is L0 again.  Every pass therefore reads its
+// lines contiguously and writes them with the element stride of one axis
+// length, lines adjacent (no pass sees the block's largest stride on both
+// sides, as the axis-by-axis plan's last pass does).  Leading untouched
+// axes stay outermost.
+void NDBackend::plan_rotated(const std::vector<int>& blk, int ndim, const int* dims) {
+  const int m = (int)blk.size();
+  int64_t M = 1;
+  for (int ax : blk) M *= dims[ax];
+  const int64_t outer = elems_ / M;
+  std::vector<int> cur(m);
+  for (int i = 0; i < m; ++i) cur[i] = i;  // layout: block axis indices, outermost first
+  auto strides = [&](const std::vector<int>& lay) {
+    std::vector<int64_t> st(m);
+    int64_t acc = 1;
+    for (int i = m - 1; i >= 0; --i) {
+      st[lay[i]] = acc;
+      acc *= dims[blk[lay[i]]];
+    }
+    return st;
+  };
+  for (int t = 0; t < m; ++t) {
+    std::vector<int> nxt(cur.begin() + 1, cur.end());
+    nxt.push_back(cur[0]);
+    const std::vector<int64_t> si = strides(cur), so = strides(nxt);
+    const int x = cur.back();
+    std::vector<Digit> lines;
+    if (outer > 1) lines.push_back(Digit{outer, M, M});
+    for (int i = 0; i + 1 < m; ++i) {
+      const int y = cur[i];
+      lines.push_back(Digit{dims[blk[y]], si[y], so[y]});
+    }
+    prog_.add_transform(dims[blk[x]], lines, si[x], so[x]);
+    cur = nxt;
+  }
+  (void)ndim;
 }
 
 NDBackend::NDBackend(int nta, const int* ta, int ndim, const int* dims, int prec)
@@ -30,15 +87,31 @@ NDBackend::NDBackend(int nta, const int* ta, int ndim, const int* dims, int prec
       fail(ERR_INVALID_DIM, "transform axes must be consecutive");
   }
   for (int i = 0; i < ndim; ++i) elems_ *= dims[i];
-  // Fastest axis first: the contiguous pass runs on the freshest data.
-  for (int j = nta - 1; j >= 0; --j) {
-    const int ax = axes[j];
-    int64_t outer = 1, inner = 1;
-    for (int i = 0; i < ax; ++i) outer *= dims[i];
-    for (int i = ax + 1; i < ndim; ++i) inner *= dims[i];
-    const int64_t n = dims[ax];
-    std::vector<Digit> lines = {{outer, n * inner, n * inner}, {inner, 1, 1}};
-    prog_.add_transform(n, lines, inner, inner);
+  // Transformed axes of length > 1 (a length-1 axis moves no data).
+  std::vector<int> blk;
+  for (int ax : axes)
+    if (dims[ax] > 1) blk.push_back(ax);
+  const bool last_axis = axes.back() == ndim - 1;
+  bool rotate = false;
+  if (last_axis && blk.size() >= 2) {
+    const int mode = rotate_mode();
+    int64_t outer_stride = 1;
+    for (int i = blk[0] + 1; i < ndim; ++i) outer_stride *= dims[i];
+    rotate = mode == 1 || (mode < 0 && prec == 0 && outer_stride * 8 >= (int64_t(4) << 20));
+  }
+  if (rotate) {
+    plan_rotated(blk, ndim, dims);
+  } else {
+    // Fastest axis first: the contiguous pass runs on the freshest data.
+    for (int j = nta - 1; j >= 0; --j) {
+      const int ax = axes[j];
+      int64_t outer = 1, inner = 1;
+      for (int i = 0; i < ax; ++i) outer *= dims[i];
+      for (int i = ax + 1; i < ndim; ++i) inner *= dims[i];
+      const int64_t n = dims[ax];
+      std::vector<Digit> lines = {{outer, n * inner, n * inner}, {inner, 1, 1}};
+      prog_.add_transform(n, lines, inner, inner);
+    }
   }
   map_pp_ = prog_.assign(PassProgram::PINGPONG, elems_);
   map_ip_ = prog_.assign(PassProgram::INPLACE, elems_);

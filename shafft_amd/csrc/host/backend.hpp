@@ -39,6 +39,8 @@ class NDBackend {
   int64_t table_bytes() const { return prog_.table_bytes(); }
 
  private:
+  static int rotate_mode();  // SHAFFT_AMD_ROTATE: 0 never, 1 always, -1 automatic
+  void plan_rotated(const std::vector<int>& blk, int ndim, const int* dims);
   int prec_;
   int64_t elems_ = 1;
   PassProgram prog_;

@@ -118,11 +118,13 @@ def test_nd_strided_axis_split(emu_lib):
     """A strided axis longer than the strided-line cap (512) is split
     (four-step, or a four-pass Bluestein chain for a prime), with the
     unit-stride row digit kept innermost so every pass is line-fast (C
-    adjacent rows per element) on both sides."""
+    adjacent rows per element) on both sides.  (Axis blocks that do not
+    include the last axis: those keep the axis-by-axis plan.)"""
     L = emu_lib
-    for dims, prec, npass in [((1200, 6), 0, 3), ((6, 700, 5), 1, 4), ((1031, 4), 1, 5), ((600, 22), 0, 3)]:
+    for dims, axes, prec, npass in [((1200, 6), [0], 0, 2), ((6, 700, 5), [0, 1], 1, 3),
+                                    ((1031, 4), [0], 1, 4), ((600, 22), [0], 0, 2)]:
         x = oracle.uniform_input(dims, 7, DT[prec])
-        p = api.NDBackendPlan(dims, list(range(len(dims))), prec, L=L)
+        p = api.NDBackendPlan(dims, axes, prec, L=L)
         assert p.num_passes() == npass, (dims, p.describe())
         for line in p.describe().strip().splitlines():
             if "iks=1 " not in line or "oks=1 " not in line:
@@ -132,8 +134,50 @@ def test_nd_strided_axis_split(emu_lib):
             a.upload(x.ravel())
             dd, _ = p.execute(a.ptr, b.ptr, d)
             y = api.download_ptr(dd, x.size, prec, L).reshape(dims)
-            ref = oracle.dftn(x.astype(np.complex128), list(range(len(dims))), sign)
+            ref = oracle.dftn(x.astype(np.complex128), axes, sign)
             assert oracle.check_rel_error(y, ref, TOL[prec]), (dims, d, oracle.rel_l2(y, ref))
+
+
+ROT_SUB = r"""
+import sys, numpy as np
+sys.path.insert(0, '.')
+import oracle
+from shafft_amd import api
+L = api.bind('_build/emu/libshafft_emu.so')
+cases = [((16, 12, 10), [0, 1, 2], 1), ((3, 8, 1, 6), [1, 2, 3], 0), ((2, 5, 7, 9, 4), [1, 2, 3, 4], 1),
+         ((33, 20), [0, 1], 0), ((5, 1, 1), [0, 1, 2], 1), ((4, 600, 10), [1, 2], 0)]
+for dims, axes, prec in cases:
+    dt = np.complex64 if prec == 0 else np.complex128
+    x = oracle.uniform_input(dims, 9, dt)
+    p = api.NDBackendPlan(list(dims), axes, prec, L=L)
+    for line in p.describe().strip().splitlines():
+        if "twiddle" not in line and "bluestein" not in line:
+            assert "iks=1 " in line, (dims, line)
+    a, b = api.DeviceBuffer(x.size, prec, L), api.DeviceBuffer(x.size, prec, L)
+    for d, sign in ((0, -1), (1, 1)):
+        a.upload(x.ravel())
+        dd, _ = p.execute(a.ptr, b.ptr, d)
+        y = api.download_ptr(dd, x.size, prec, L).reshape(dims)
+        ref = oracle.dftn(x.astype(np.complex128), axes, sign)
+        assert oracle.check_rel_error(y, ref, 1e-5 if prec == 0 else 1e-12), (dims, axes, d)
+    print(dims, p.num_passes())
+"""
+
+
+def test_nd_rotated_plans(emu_lib):
+    """Axis blocks ending at the last axis can rotate the layout one axis per
+    pass (forced here with SHAFFT_AMD_ROTATE=1; automatic for complex64
+    outermost strides of 4 MiB or more): every pass reads its lines
+    contiguously (iks = 1) and the last pass restores row-major order.
+    Length-1 axes are skipped; leading untouched axes stay outermost."""
+    out = _run_in_subprocess(ROT_SUB, {"SHAFFT_AMD_ROTATE": "1"})
+    assert "(16, 12, 10) 3" in out and "(5, 1, 1) 1" in out
+    L = emu_lib
+    # automatic choice: 1024^3 complex64 rotates, 512^3 complex128 does not
+    p = api.NDBackendPlan([1024] * 3, [0, 1, 2], 0, L=L)
+    assert all("iks=1 " in ln for ln in p.describe().strip().splitlines())
+    q = api.NDBackendPlan([512] * 3, [0, 1, 2], 1, L=L)
+    assert "iks=262144" in q.describe()
 
 
 def test_nd_small_strided_cap(emu_lib):
