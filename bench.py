@@ -251,6 +251,12 @@ def bench_fft1d(args, hip):
     a, b = api.DeviceBuffer(ext, prec), api.DeviceBuffer(ext, prec)
     a.upload(x)
     del x
+    if args.calibrate_copy:
+        esz = 16 if prec == api.Z2Z else 8
+        hip.L.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+        assert hip.L.hipMemcpy(ct.c_void_p(b.ptr), ct.c_void_p(a.ptr), ct.c_size_t(n * esz), 3) == 0
+        hip.sync()
+        print(f"calibration copy: {n * esz} bytes read, {n * esz} bytes written", file=sys.stderr)
     for _ in range(args.warmup):
         p.execute(a.ptr, b.ptr, api.FORWARD)
     p.synchronize()
