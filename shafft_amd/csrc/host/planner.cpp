@@ -329,6 +329,7 @@ static bool configure_fixed(sa_pass_t& p, int prec, int64_t& grid) {
     p.tw_lds = (int32_t)tw_elems;
     lds += (size_t)tw_elems * esz;
   }
+  p.xcd_order = env_int("SHAFFT_AMD_XCD", 0) ? 1 : 0;
   const int occ = sa_fixed_occupancy(n, prec, -1, nthr, lds);
   if (occ <= 0) return false;
   const int64_t tiles = (p.batch + (1 << logC) - 1) >> logC;

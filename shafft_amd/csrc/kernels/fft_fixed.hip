@@ -329,13 +329,22 @@ __global__ __launch_bounds__(512) void fft_fixed_pass(const sa_pass_t a) {
   cpx<R>* line = lds + (size_t)c * stride;
 
   const uint32_t tiles = (uint32_t)((a.batch + C - 1) >> logC);
-  uint32_t tile = blockIdx.x;
+  // Tile order: workgroup b walks b, b+G, ... (workgroups are dealt to the
+  // 8 XCDs round robin, so each XCD sees every 8th tile), or with xcd_order
+  // the workgroups of XCD x = b % 8 walk their own contiguous eighth.
+  uint32_t tile = blockIdx.x, step = gridDim.x, end = tiles;
+  if (a.xcd_order && (gridDim.x & 7u) == 0) {
+    const uint32_t x = blockIdx.x & 7u, per = (tiles + 7u) >> 3;
+    tile = x * per + (blockIdx.x >> 3);
+    step = gridDim.x >> 3;
+    end = min(tiles, (x + 1u) * per);
+  }
   cpx<R> v[E];
-  if (tile < tiles) load_tile<R, N>(a, (tile << logC) + c, tl, v);
-  while (tile < tiles) {
+  if (tile < end) load_tile<R, N>(a, (tile << logC) + c, tl, v);
+  while (tile < end) {
     stage_first<R, N, SIGN>(v, line, tl);
-    const uint32_t next = tile + gridDim.x;
-    if (next < tiles) load_tile<R, N>(a, (next << logC) + c, tl, v);  // prefetch
+    const uint32_t next = tile + step;
+    if (next < end) load_tile<R, N>(a, (next << logC) + c, tl, v);  // prefetch
     mids<R, N, SIGN, 1>(line, a, twb, tl);
     __syncthreads();
     if (same) {
