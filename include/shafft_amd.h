@@ -230,6 +230,16 @@ int shafftLastErrorDomainCode(void);                                  /* shafft.
 int shafftLastErrorMessage(char* buf, int buflen);                    /* shafft.h:175 */
 void shafftClearLastError(void);                                      /* shafft.h:183 */
 const char* shafftErrorSourceName(int source);                        /* shafft.h:191 */
+typedef enum { SHAFFT_MAXIMIZE_NDA, SHAFFT_MINIMIZE_NDA } shafft_decomposition_strategy_t;
+/* One rank: the grid resolves to {1,...,1}, nda 0, subsize = size, offset 0,
+ * commSize 1, whatever commDims/nda/strategy ask for (the reference's
+ * commDims -> nda -> strategy fallback chain ends there at one rank,
+ * src/cpp/configuration.cpp:530-600); memLimit is not applied at one rank
+ * (:365, no Cartesian communicator).  ndim 1 needs commDims = {1}. */
+int shafftConfigurationND(int ndim, int* size, shafft_t precision, int* commDims, int* nda,
+                          size_t* subsize, size_t* offset, int* commSize,
+                          shafft_decomposition_strategy_t strategy, size_t memLimit,
+                          shafft_comm_t cComm);                       /* shafft.h:510 */
 int shafftNDCreate(void** outPlan);                                   /* shafft.h:529 */
 int shafftNDInit(void* planPtr, int ndim, int commDims[], int dimensions[], shafft_t precision,
                  shafft_comm_t cComm, shafft_transform_layout_t output_policy); /* :544 */
@@ -263,6 +273,24 @@ const char* shafftGetBackendName(void);                               /* shafft.
 void shafftGetVersion(int* major, int* minor, int* patch);            /* shafft.h:827 */
 const char* shafftGetVersionString(void);                             /* shafft.h:834 */
 int shafftFinalize(void);                                             /* shafft.h:845 */
+
+/* Fortran-facing companions bound by the reference's Fortran module
+ * (src/f03/shafftf03.h; src/f03/shafft.f03).  The Fortran communicator
+ * handle is read through the pointer and must be 0 (the one-rank handle,
+ * as cComm above); everything else forwards to the C entry point. */
+typedef int shafft_fint_t; /* MPI_Fint */
+int shafftConfigurationNDf03(int ndim, int* size, shafft_t precision, int* commDims, int* nda,
+                             size_t* subsize, size_t* offset, int* commSize,
+                             shafft_decomposition_strategy_t strategy, size_t memLimit,
+                             shafft_fint_t* fHandle);                 /* shafftf03.h:11 */
+int shafftNDInitf03(void* planPtr, int ndim, int commDims[], int dimensions[],
+                    shafft_t precision, shafft_fint_t* fHandle,
+                    shafft_transform_layout_t* output_policy);        /* shafftf03.h:23 */
+int shafftConfiguration1Df03(size_t N, size_t* localN, size_t* localStart,
+                             size_t* localAllocSize, shafft_t precision,
+                             shafft_fint_t* fHandle);                 /* shafftf03.h:32 */
+int shafft1DInitf03(void* planPtr, size_t N, size_t localN, size_t localStart,
+                    shafft_t precision, shafft_fint_t* fHandle);      /* shafftf03.h:39 */
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -115,6 +115,19 @@ _SIGS = {
                                          ct.POINTER(ct.c_size_t), ct.c_int, ct.c_ssize_t]),
     "shafft1DCreate": (ct.c_int, [ct.POINTER(ct.c_void_p)]),
     "shafft1DInit": (ct.c_int, [ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_int, ct.c_ssize_t]),
+    "shafftConfigurationND": (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.c_int, ct.POINTER(ct.c_int),
+                                         ct.POINTER(ct.c_int), ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t),
+                                         ct.POINTER(ct.c_int), ct.c_int, ct.c_size_t, ct.c_ssize_t]),
+    "shafftConfigurationNDf03": (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.c_int, ct.POINTER(ct.c_int),
+                                            ct.POINTER(ct.c_int), ct.POINTER(ct.c_size_t),
+                                            ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_int), ct.c_int,
+                                            ct.c_size_t, ct.POINTER(ct.c_int)]),
+    "shafftNDInitf03": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), ct.c_int,
+                                   ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
+    "shafftConfiguration1Df03": (ct.c_int, [ct.c_size_t, ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t),
+                                            ct.POINTER(ct.c_size_t), ct.c_int, ct.POINTER(ct.c_int)]),
+    "shafft1DInitf03": (ct.c_int, [ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_int,
+                                   ct.POINTER(ct.c_int)]),
     "shafftAllocBufferF": (ct.c_int, [ct.c_size_t, ct.POINTER(ct.c_void_p)]),
     "shafftAllocBufferD": (ct.c_int, [ct.c_size_t, ct.POINTER(ct.c_void_p)]),
     "shafftFreeBufferF": (ct.c_int, [ct.c_void_p]),

@@ -654,6 +654,80 @@ int shafftConfiguration1D(size_t N, size_t* localN, size_t* localStart, size_t* 
   });
 }
 
+// One-rank outcome of the commDims -> nda -> strategy chain
+// (src/cpp/configuration.cpp:530-600): a grid of ones (:478-498) or, for any
+// other request, the automatic search, whose only candidate at one rank packs
+// to d = 0 (:318-384) and so needs ndim >= 2.
+int shafftConfigurationND(int ndim, int* size, shafft_t precision, int* commDims, int* nda,
+                          size_t* subsize, size_t* offset, int* commSize,
+                          shafft_decomposition_strategy_t strategy, size_t memLimit,
+                          shafft_comm_t cComm) {
+  (void)memLimit;
+  return guarded([&] {
+    (void)prec_of(precision);
+    check_comm(cComm);
+    if (!size || !commDims || !nda || !subsize || !offset || !commSize)
+      fail(ERR_NULLPTR, "null argument");
+    if (ndim <= 0) fail(ERR_INVALID_DIM, "ndim must be positive");
+    if (strategy != SHAFFT_MAXIMIZE_NDA && strategy != SHAFFT_MINIMIZE_NDA)
+      strategy = SHAFFT_MINIMIZE_NDA;  // shafftc.cpp:130-132
+    for (int i = 0; i < ndim; ++i)
+      if (size[i] <= 0) fail(ERR_INVALID_DIM, "dimension must be positive");
+    bool ones = true;
+    for (int i = 0; i < ndim; ++i) ones = ones && commDims[i] == 1;
+    if (!ones && ndim < 2) fail(ERR_INVALID_DECOMP, "no decomposition of a 1-D tensor");
+    for (int i = 0; i < ndim; ++i) {
+      commDims[i] = 1;
+      subsize[i] = (size_t)size[i];
+      offset[i] = 0;
+    }
+    *nda = 0;
+    *commSize = 1;
+    return 0;
+  });
+}
+
+// Fortran-facing companions (src/f03/shafftf03.cpp:11-60): the Fortran handle
+// arrives by reference and is read as the one-rank handle value.
+int shafftConfigurationNDf03(int ndim, int* size, shafft_t precision, int* commDims, int* nda,
+                             size_t* subsize, size_t* offset, int* commSize,
+                             shafft_decomposition_strategy_t strategy, size_t memLimit,
+                             shafft_fint_t* fHandle) {
+  return guarded([&]() -> int {
+    if (!fHandle) fail(ERR_NULLPTR, "null communicator handle");
+    return shafftConfigurationND(ndim, size, precision, commDims, nda, subsize, offset, commSize,
+                                 strategy, memLimit, (shafft_comm_t)*fHandle);
+  });
+}
+
+int shafftNDInitf03(void* planPtr, int ndim, int commDims[], int dimensions[],
+                    shafft_t precision, shafft_fint_t* fHandle,
+                    shafft_transform_layout_t* output_policy) {
+  return guarded([&]() -> int {
+    if (!fHandle || !output_policy) fail(ERR_NULLPTR, "null communicator handle or output policy");
+    return shafftNDInit(planPtr, ndim, commDims, dimensions, precision, (shafft_comm_t)*fHandle,
+                        *output_policy);
+  });
+}
+
+int shafftConfiguration1Df03(size_t N, size_t* localN, size_t* localStart,
+                             size_t* localAllocSize, shafft_t precision,
+                             shafft_fint_t* fHandle) {
+  return guarded([&]() -> int {
+    if (!fHandle) fail(ERR_NULLPTR, "null communicator handle");
+    return shafftConfiguration1D(N, localN, localStart, localAllocSize, precision,
+                                 (shafft_comm_t)*fHandle);
+  });
+}
+
+int shafft1DInitf03(void* planPtr, size_t N, size_t localN, size_t localStart,
+                    shafft_t precision, shafft_fint_t* fHandle) {
+  return guarded([&]() -> int {
+    if (!fHandle) fail(ERR_NULLPTR, "null communicator handle");
+    return shafft1DInit(planPtr, N, localN, localStart, precision, (shafft_comm_t)*fHandle);
+  });
+}
+
 int shafftPlan(void* planPtr) {
   return guarded([&] {
     Plan* p = as_plan(planPtr);

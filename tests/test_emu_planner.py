@@ -266,6 +266,68 @@ def test_c_api_state_machine_and_errors(emu_lib):
     assert L.shafftDestroy(ct.byref(h1)) == 0
 
 
+def test_c_api_configuration_nd_and_fortran_entry_points(emu_lib):
+    """shafftConfigurationND at one rank (src/cpp/configuration.cpp:530-600:
+    a grid of ones is taken as given, :478-498; any other request falls
+    through to the automatic search, which packs to nda 0 and has no candidate
+    for a 1-D tensor, :318-410) and the Fortran-handle companions
+    (src/f03/shafftf03.cpp:11-60)."""
+    L = emu_lib
+    import ctypes as ct
+    Z, ONE = ct.c_size_t(0), ct.c_size_t(1)   # size_t / intptr_t arguments on the stack
+    C0, C3 = ct.c_ssize_t(0), ct.c_ssize_t(3)
+    size = (ct.c_int * 3)(64, 48, 32)
+    sub, off = (ct.c_size_t * 3)(), (ct.c_size_t * 3)()
+    nda, cs = ct.c_int(), ct.c_int()
+    for grid, want_nda in (((1, 1, 1), 0), ((4, 2, 1), 0), ((0, 0, 0), 0)):
+        cd = (ct.c_int * 3)(*grid)
+        nda.value = 2
+        assert L.shafftConfigurationND(3, size, 1, cd, ct.byref(nda), sub, off, ct.byref(cs),
+                                       0, ONE, C0) == 0   # memLimit 1 byte: not applied at 1 rank
+        assert list(cd) == [1, 1, 1] and nda.value == want_nda and cs.value == 1
+        assert list(sub) == [64, 48, 32] and list(off) == [0, 0, 0]
+    one = (ct.c_int * 1)(100)
+    s1, o1 = (ct.c_size_t * 1)(), (ct.c_size_t * 1)()
+    cd1 = (ct.c_int * 1)(0)
+    assert L.shafftConfigurationND(1, one, 0, cd1, ct.byref(nda), s1, o1, ct.byref(cs), 1, Z, C0) == 7
+    cd1[0] = 1
+    assert L.shafftConfigurationND(1, one, 0, cd1, ct.byref(nda), s1, o1, ct.byref(cs), 1, Z, C0) == 0
+    assert s1[0] == 100 and cs.value == 1
+    cd = (ct.c_int * 3)(1, 1, 1)
+    assert L.shafftConfigurationND(3, size, 9, cd, ct.byref(nda), sub, off, ct.byref(cs), 0, Z, C0) == 8
+    assert L.shafftConfigurationND(3, size, 0, cd, ct.byref(nda), sub, off, ct.byref(cs), 0, Z, C3) == 2
+    bad = (ct.c_int * 3)(64, 0, 32)
+    assert L.shafftConfigurationND(3, bad, 0, cd, ct.byref(nda), sub, off, ct.byref(cs), 0, Z, C0) == 5
+    assert L.shafftConfigurationND(3, size, 0, cd, None, sub, off, ct.byref(cs), 0, Z, C0) == 1
+
+    # Fortran companions: handle by reference, 0 = the one-rank handle.
+    fh, fbad = ct.c_int(0), ct.c_int(1140850688)
+    assert L.shafftConfigurationNDf03(3, size, 0, cd, ct.byref(nda), sub, off, ct.byref(cs),
+                                      1, Z, ct.byref(fh)) == 0
+    assert L.shafftConfigurationNDf03(3, size, 0, cd, ct.byref(nda), sub, off, ct.byref(cs),
+                                      1, Z, ct.byref(fbad)) == 2
+    assert L.shafftConfigurationNDf03(3, size, 0, cd, ct.byref(nda), sub, off, ct.byref(cs),
+                                      1, Z, None) == 1
+    h = ct.c_void_p()
+    assert L.shafftNDCreate(ct.byref(h)) == 0
+    pol = ct.c_int(1)
+    assert L.shafftNDInitf03(h, 3, cd, size, 0, ct.byref(fh), None) == 1
+    assert L.shafftNDInitf03(h, 3, cd, size, 0, ct.byref(fbad), ct.byref(pol)) == 2
+    assert L.shafftNDInitf03(h, 3, cd, size, 0, ct.byref(fh), ct.byref(pol)) == 0
+    n = ct.c_size_t()
+    assert L.shafftGetAllocSize(h, ct.byref(n)) == 0 and n.value == 64 * 48 * 32
+    assert L.shafftDestroy(ct.byref(h)) == 0
+    ln, ls, la = ct.c_size_t(), ct.c_size_t(), ct.c_size_t()
+    assert L.shafftConfiguration1Df03(77, ct.byref(ln), ct.byref(ls), ct.byref(la), 1,
+                                      ct.byref(fh)) == 0
+    assert (ln.value, ls.value, la.value) == (77, 0, 77)
+    h1 = ct.c_void_p()
+    assert L.shafft1DCreate(ct.byref(h1)) == 0
+    assert L.shafft1DInitf03(h1, 77, 77, 0, 1, ct.byref(fbad)) == 2
+    assert L.shafft1DInitf03(h1, 77, 77, 0, 1, ct.byref(fh)) == 0
+    assert L.shafftDestroy(ct.byref(h1)) == 0
+
+
 def test_c_api_roundtrip_and_normalize(emu_lib):
     L = emu_lib
     for prec in (0, 1):
