@@ -273,6 +273,15 @@ static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
   p.reg_first = regs ? 1 : 0;
   p.reg_last = regs ? 1 : 0;
   p.nt = env_int("SHAFFT_AMD_NT", 1) ? 1 : 0;
+  // Stage twiddles staged in LDS behind the tile when they fit (not for
+  // Bluestein passes, whose two stage sweeps keep reading global memory).
+  int64_t tw_elems = 0;
+  for (int s = 1; s < p.nstages; ++s)
+    tw_elems = std::max<int64_t>(tw_elems, (int64_t)p.twoff[s] + (int64_t)p.ns[s] * (p.radix[s] - 1));
+  p.tw_lds = 0;
+  if (!bluestein && env_int("SHAFFT_AMD_GTW_LDS", 1) && tw_elems > 0 &&
+      ((int64_t)C * stride + tw_elems) * elem_bytes <= budget)
+    p.tw_lds = (int32_t)tw_elems;
 }
 
 static int device_cus() {
@@ -323,15 +332,16 @@ static bool configure_fixed(sa_pass_t& p, int prec, int64_t& grid) {
   int64_t tw_elems = 0;
   for (int s = 1; s < p.nstages; ++s)
     tw_elems = std::max<int64_t>(tw_elems, (int64_t)p.twoff[s] + (int64_t)p.ns[s] * (p.radix[s] - 1));
-  p.tw_lds = 0;
+  int32_t tw_lds = 0;
   if (env_int("SHAFFT_AMD_TW_LDS", 1) && tw_elems > 0 &&
       lds + (size_t)tw_elems * esz <= (size_t)budget) {
-    p.tw_lds = (int32_t)tw_elems;
+    tw_lds = (int32_t)tw_elems;
     lds += (size_t)tw_elems * esz;
   }
-  p.xcd_order = env_int("SHAFFT_AMD_XCD", 0) ? 1 : 0;
   const int occ = sa_fixed_occupancy(n, prec, -1, nthr, lds);
-  if (occ <= 0) return false;
+  if (occ <= 0) return false;  // p keeps the generic kernel's launch shape
+  p.tw_lds = tw_lds;
+  p.xcd_order = env_int("SHAFFT_AMD_XCD", 0) ? 1 : 0;
   const int64_t tiles = (p.batch + (1 << logC) - 1) >> logC;
   grid = std::min<int64_t>(tiles, (int64_t)occ * device_cus());
   p.logC = logC;

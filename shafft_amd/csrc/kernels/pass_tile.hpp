@@ -202,9 +202,10 @@ __device__ __forceinline__ void first_dispatch(const sa_pass_t& a, const cpx<R> 
 
 template <typename R, int SIGN>
 __device__ __forceinline__ void last_dispatch(const sa_pass_t& a, cpx<R> (&v)[EMAX],
-                                              const cpx<R>* line, int tl, bool act) {
+                                              const cpx<R>* line, int tl, bool act,
+                                              const cpx<R>* twb) {
   const int s = a.nstages - 1;
-  const cpx<R>* tws = reinterpret_cast<const cpx<R>*>(a.tw) + a.twoff[s];
+  const cpx<R>* tws = twb + a.twoff[s];
   switch (a.radix[s]) {
     case 2: stage_last_regs<R, 2, SIGN>(v, line, a.n, tws, tl, a.T, act); break;
     case 4: stage_last_regs<R, 4, SIGN>(v, line, a.n, tws, tl, a.T, act); break;
@@ -360,9 +361,21 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
 // stage path (both flags set, same lane mapping on both sides), Bluestein.
 enum PassMode { MODE_LDS = 0, MODE_REGS = 1, MODE_BLUE = 2 };
 
+// Copies the stage twiddle table (a.tw_lds elements) into LDS after the C
+// line slots; the callers place it after the tile's HBM loads are issued, so
+// its L2 latency overlaps theirs instead of returning in every stage.
+template <typename R>
+__device__ __forceinline__ const cpx<R>* stage_twiddles_to_lds(const sa_pass_t& a, cpx<R>* lds) {
+  const cpx<R>* __restrict__ g = reinterpret_cast<const cpx<R>*>(a.tw);
+  cpx<R>* ltw = lds + (size_t)a.C * a.ldsstride;
+  for (int i = (int)threadIdx.x; i < a.tw_lds; i += (int)blockDim.x) ltw[i] = g[i];
+  return ltw;  // read after the next barrier
+}
+
 // One tile (C consecutive lines) of a pass, run by the whole workgroup.
 // Threads t >= C*T of a larger workgroup idle through the barriers.
-template <typename R, int SIGN, int MODE, int MX = 0, bool R16 = true>
+// TWL: the stages read their twiddles from LDS (a.tw_lds > 0, not Bluestein).
+template <typename R, int SIGN, int MODE, int MX = 0, bool R16 = true, bool TWL = false>
 __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx<R>* lds) {
   constexpr bool BLUE = MODE == MODE_BLUE;
 
@@ -382,6 +395,8 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   const uint32_t tl = active ? slot_elem(a, t, id_kfast) : (uint32_t)n;
   cpx<R>* line = lds + (size_t)c * a.ldsstride;
   cpx<R> v[EMAX];
+  static_assert(!(TWL && BLUE), "Bluestein passes read their twiddles from global memory");
+  const cpx<R>* twb = reinterpret_cast<const cpx<R>*>(a.tw);
 
   // ---- load ----------------------------------------------------------------
   if constexpr (reg_first) {
@@ -393,15 +408,20 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
 #pragma unroll
       for (int i = 0; i < EMAX; ++i) v[i] = {R(0), R(0)};
     }
+    if constexpr (TWL) twb = stage_twiddles_to_lds<R>(a, lds);
     first_dispatch<R, SIGN>(a, v, line, (int)tl, active);
   } else {
     const uint32_t lc = slot_line(a, t, a.ld_kfast != 0);
     const uint32_t k0 = slot_elem(a, t, a.ld_kfast != 0);
     const uint32_t b = line0 + lc;
-    if (active && b < (uint32_t)a.batch) {
+    const bool have = active && b < (uint32_t)a.batch;
+    cpx<R> u[EMAX];
+    if (have) {
       const LineOff o = line_offsets(a, b);
-      cpx<R> u[EMAX];
       load_line<R>(a, o, k0, u);
+    }
+    if constexpr (TWL) twb = stage_twiddles_to_lds<R>(a, lds);
+    if (have) {
       cpx<R>* ll = lds + (size_t)lc * a.ldsstride;
 #pragma unroll
       for (int i = 0; i < EMAX; ++i) {
@@ -416,12 +436,10 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   if constexpr (!BLUE) {
     const int s0 = reg_first ? 1 : 0;
     const int s1 = reg_last ? uni(a.nstages) - 1 : uni(a.nstages);
-    const cpx<R>* twb = reinterpret_cast<const cpx<R>*>(a.tw);
     for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX, R16>(a, s, line, (int)tl, lds, twb);
-    if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl, active);
+    if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl, active, twb);
   } else {
     const int nst = uni(a.nstages);
-    const cpx<R>* twb = reinterpret_cast<const cpx<R>*>(a.tw);
     for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0, R16>(a, s, line, (int)tl, lds, twb);
     const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
     for (int k = (int)tl; k < n; k += a.T) line[lp(k)] = cmul(line[lp(k)], h[k]);

@@ -123,7 +123,16 @@ void check_shape(const sa_pass_t& p) {
     const int id_kfast = p.reg_first ? p.ld_kfast : p.st_kfast;
     if (p.reg_last && id_kfast != p.st_kfast) die("reg_last with a different store mapping");
   }
-  if ((int64_t)p.C * p.ldsstride * (int64_t)sizeof(C<R>) > 160 * 1024) die("lds bytes");
+  if (((int64_t)p.C * p.ldsstride + p.tw_lds) * (int64_t)sizeof(C<R>) > 160 * 1024) die("lds bytes");
+  if (p.tw_lds < 0) die("tw_lds");
+  if (p.tw_lds > 0) {
+    if (p.blu_h) die("staged twiddles on a Bluestein pass");
+    int64_t need = 0;  // the staged table must cover every twiddle a stage reads
+    for (int s = 1; s < p.nstages; ++s)
+      need = std::max<int64_t>(need, (int64_t)p.twoff[s] + (int64_t)p.ns[s] * (p.radix[s] - 1));
+    if (p.tw_lds < need) die("tw_lds below the stage table");
+    chk("tw", p.tw, p.tw_lds - 1, sizeof(C<R>));
+  }
   if (p.batch >= (int64_t(1) << 31)) die("batch");
   if (p.ptw_level > 2 || p.ld_level > 2 || p.st_level > 2 || p.ld_level2 > 2 || p.st_level2 > 2)
     die("level");
