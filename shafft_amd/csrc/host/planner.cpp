@@ -251,7 +251,10 @@ static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
     // line-fast lanes walk lines: keep C within the fast digit
     C = (int)std::min<int64_t>(C, p.n2);
   }
-  while (C > 1 && C * T > 512) C /= 2;
+  // At most 256 threads per tile (unless one line needs more): several
+  // tiles per CU overlap loads with stages (measured best of 128..512).
+  const int max_thr = std::max(T, std::min(512, env_int("SHAFFT_AMD_WG_THREADS", 256)));
+  while (C > 1 && C * T > max_thr) C /= 2;
   // padded row: index k -> k + k/16, odd stride so line-fast lanes spread
   int stride = n + (n - 1) / 16;
   if (stride % 2 == 0) stride += 1;

@@ -20,24 +20,15 @@
 
 namespace sa {
 
-// Register budget.  The complex-float kernel runs plans of radix 8 at most
-// (the radix-16 butterfly sets the whole kernel's register footprint) and is
-// capped at 128 VGPRs (4 waves per SIMD): two workgroups then share a CU and
-// one's HBM loads overlap the other's LDS stages; the few spills this costs
-// are cheaper (2.08 vs 1.90 TB/s over the serving stream's 30 largest
-// complex64 jobs).  The complex-double kernel spills too much under that cap
-// (1.96 vs 3.04 TB/s) and keeps 256 VGPRs and radix 16.
-// (The cap applies to plain LDS-mode passes without element hooks; the
-// register-stage and Bluestein modes and hooked passes hold more per thread
-// and run uncapped.)
-template <int SIGN, bool TWL>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4)))
-void fft_lds_pass_fc(const sa_pass_t a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<float, SIGN, MODE_LDS, 0, false, TWL>(a, blockIdx.x,
-                                                  reinterpret_cast<cpx<float>*>(lds_raw));
-}
-
+// Register budget and workgroup size.  The complex-float kernel runs plans
+// of radix 8 at most (the radix-16 butterfly sets the whole kernel's
+// register footprint: ~155 VGPRs without it, 3 waves per SIMD); the
+// complex-double kernel keeps radix 16.  Occupancy comes from workgroups of
+// at most 256 threads (the planner's tile choice): several independent tiles
+// per CU overlap one's HBM loads with another's LDS stages.  Measured over
+// the 30 largest jobs of each precision of the serving stream: complex64
+// 2.18 -> 2.70 TB/s, complex128 3.31 -> 3.76 TB/s against 512-thread tiles; a
+// 128-VGPR cap (4 waves per SIMD, with spills) lost to it at every tile size.
 template <int SIGN, int MODE, bool TWL>
 __global__ __launch_bounds__(512) void fft_lds_pass_f(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -50,18 +41,14 @@ __global__ __launch_bounds__(512) void fft_lds_pass_d(const sa_pass_t a) {
   pass_tile<double, SIGN, MODE, 0, true, TWL>(a, blockIdx.x, reinterpret_cast<cpx<double>*>(lds_raw));
 }
 
-template <typename R, int SIGN, int MODE, bool CAP, bool TWL>
+template <typename R, int SIGN, int MODE, bool TWL>
 struct PassKernel;
-template <int SIGN, int MODE, bool CAP, bool TWL>
-struct PassKernel<float, SIGN, MODE, CAP, TWL> {
-  static const void* fn() {
-    if constexpr (MODE == MODE_LDS && CAP)
-      return reinterpret_cast<const void*>(&fft_lds_pass_fc<SIGN, TWL>);
-    return reinterpret_cast<const void*>(&fft_lds_pass_f<SIGN, MODE, TWL>);
-  }
+template <int SIGN, int MODE, bool TWL>
+struct PassKernel<float, SIGN, MODE, TWL> {
+  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_f<SIGN, MODE, TWL>); }
 };
-template <int SIGN, int MODE, bool CAP, bool TWL>
-struct PassKernel<double, SIGN, MODE, CAP, TWL> {
+template <int SIGN, int MODE, bool TWL>
+struct PassKernel<double, SIGN, MODE, TWL> {
   static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_d<SIGN, MODE, TWL>); }
 };
 
@@ -77,44 +64,36 @@ __global__ __launch_bounds__(256) void scale_kernel(cpx<R>* __restrict__ x, R f,
 }
 
 // TWL: the stage twiddles are staged in LDS behind the tile (a.tw_lds > 0).
-template <typename R, int SIGN, int MODE, bool CAP, bool TWL>
+template <typename R, int SIGN, int MODE, bool TWL>
 static hipError_t launch_t(const sa_pass_t* p, hipStream_t st) {
   const uint32_t nthr = (uint32_t)(p->C * p->T);
   const uint64_t nblk = ((uint64_t)p->batch + p->C - 1) / p->C;
   const size_t lds = ((size_t)p->C * p->ldsstride + (TWL ? (size_t)p->tw_lds : 0)) * sizeof(cpx<R>);
   static const bool attr = [] {  // once per instantiation, thread-safe
-    (void)hipFuncSetAttribute(PassKernel<R, SIGN, MODE, CAP, TWL>::fn(),
+    (void)hipFuncSetAttribute(PassKernel<R, SIGN, MODE, TWL>::fn(),
                               hipFuncAttributeMaxDynamicSharedMemorySize, sa_max_lds_bytes());
     (void)hipGetLastError();
     return true;
   }();
   (void)attr;
-  if constexpr (sizeof(R) == 4 && MODE == MODE_LDS && CAP)
-    hipLaunchKernelGGL((fft_lds_pass_fc<SIGN, TWL>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
-  else if constexpr (sizeof(R) == 4)
+  if constexpr (sizeof(R) == 4)
     hipLaunchKernelGGL((fft_lds_pass_f<SIGN, MODE, TWL>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
   else
     hipLaunchKernelGGL((fft_lds_pass_d<SIGN, MODE, TWL>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
   return hipGetLastError();
 }
 
-template <typename R, int SIGN, int MODE, bool CAP>
+template <typename R, int SIGN, int MODE>
 static hipError_t launch_tw(const sa_pass_t* p, hipStream_t st) {
-  return p->tw_lds > 0 ? launch_t<R, SIGN, MODE, CAP, true>(p, st)
-                       : launch_t<R, SIGN, MODE, CAP, false>(p, st);
+  return p->tw_lds > 0 ? launch_t<R, SIGN, MODE, true>(p, st) : launch_t<R, SIGN, MODE, false>(p, st);
 }
 
 template <typename R>
 static hipError_t launch_prec(const sa_pass_t* p, int sign, hipStream_t st) {
-  if (p->blu_h) return launch_t<R, -1, MODE_BLUE, false, false>(p, st);
+  if (p->blu_h) return launch_t<R, -1, MODE_BLUE, false>(p, st);
   const bool regs = p->reg_first && p->reg_last;
-  const bool hooks = p->pre || p->post || p->ld_level >= 0 || p->st_level >= 0 ||
-                     p->ld_kmul != 1 || p->st_kmul != 1 || p->ld_limit < p->n || p->st_limit < p->n;
-  if (hooks && !regs)
-    return sign < 0 ? launch_tw<R, -1, MODE_LDS, false>(p, st) : launch_tw<R, +1, MODE_LDS, false>(p, st);
-  if (sign < 0)
-    return regs ? launch_tw<R, -1, MODE_REGS, false>(p, st) : launch_tw<R, -1, MODE_LDS, true>(p, st);
-  return regs ? launch_tw<R, +1, MODE_REGS, false>(p, st) : launch_tw<R, +1, MODE_LDS, true>(p, st);
+  if (sign < 0) return regs ? launch_tw<R, -1, MODE_REGS>(p, st) : launch_tw<R, -1, MODE_LDS>(p, st);
+  return regs ? launch_tw<R, +1, MODE_REGS>(p, st) : launch_tw<R, +1, MODE_LDS>(p, st);
 }
 
 }  // namespace sa
