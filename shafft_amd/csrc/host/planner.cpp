@@ -117,12 +117,38 @@ std::vector<int> radix_plan(int64_t n, bool max8) {
     if (rem == 2) r.push_back(4);
     if (rem == 1) r.push_back(2);
   }
-  for (int p : {3, 5, 7})
-    while (n % p == 0) {
-      n /= p;
-      r.push_back(p);
-    }
+  int e3 = 0, e5 = 0, e7 = 0;
+  for (; n % 3 == 0; n /= 3) ++e3;
+  for (; n % 5 == 0; n /= 5) ++e5;
+  for (; n % 7 == 0; n /= 7) ++e7;
   if (n != 1) fail(ERR_INTERNAL, "radix_plan: length is not 7-smooth");
+  // Fewer stages = fewer LDS round trips and barriers: threes pair into
+  // radix 9, and a lone 3 or 5 merges with the trailing radix-2/4 stage
+  // into the prime-factor butterflies 6, 12 or 10.
+  const bool tail2 = !r.empty() && r.back() == 2, tail4 = !r.empty() && r.back() == 4;
+  int e9 = e3 / 2;
+  e3 %= 2;
+  int r6 = 0, r12 = 0, r10 = 0;
+  if (e3 == 1 && tail2) {
+    r.pop_back();
+    r6 = 1;
+    e3 = 0;
+  } else if (e3 == 1 && tail4) {
+    r.pop_back();
+    r12 = 1;
+    e3 = 0;
+  } else if (e5 > 0 && tail2) {
+    r.pop_back();
+    r10 = 1;
+    --e5;
+  }
+  if (r12) r.push_back(12);
+  if (r10) r.push_back(10);
+  for (int i = 0; i < e9; ++i) r.push_back(9);
+  if (r6) r.push_back(6);
+  for (int i = 0; i < e7; ++i) r.push_back(7);
+  for (int i = 0; i < e5; ++i) r.push_back(5);
+  if (e3) r.push_back(3);
   return r;
 }
 
@@ -251,9 +277,12 @@ static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
     // line-fast lanes walk lines: keep C within the fast digit
     C = (int)std::min<int64_t>(C, p.n2);
   }
-  // At most 256 threads per tile (unless one line needs more): several
-  // tiles per CU overlap loads with stages (measured best of 128..512).
-  const int max_thr = std::max(T, std::min(512, env_int("SHAFFT_AMD_WG_THREADS", 256)));
+  // Tiles of at most 64 KiB of data (512 threads for complex float, 256 for
+  // complex double; more only when one line needs it): several tiles per CU
+  // overlap one's loads with another's stages (measured best of 128..512
+  // threads over the serving stream's largest jobs, per precision).
+  const int thr_dflt = 65536 / (emax * elem_bytes);
+  const int max_thr = std::max(T, std::min(512, env_int("SHAFFT_AMD_WG_THREADS", thr_dflt)));
   while (C > 1 && C * T > max_thr) C /= 2;
   // padded row: index k -> k + k/16, odd stride so line-fast lanes spread
   int stride = n + (n - 1) / 16;

@@ -1,5 +1,5 @@
 // Device-side building blocks for the gfx950 FFT kernels: complex arithmetic,
-// fixed-radix DFT butterflies (2,3,4,5,7,8,16), and 32-bit division by a
+// fixed-radix DFT butterflies (2,3,4,5,6,7,8,9,10,12,16), and 32-bit division by a
 // run-time invariant.
 //
 // Conventions
@@ -205,11 +205,89 @@ __device__ __forceinline__ void dft_odd(cpx<R>* v) {
 }
 
 template <int RAD, int SIGN, typename R>
+__device__ __forceinline__ void dft(cpx<R>* v);
+
+// Composite radix R = R1 * R2 with gcd(R1, R2) = 1 by the prime-factor
+// (Good-Thomas) map: input n = (R2 n1 + R1 n2) mod R, output k with
+// k = k1 (mod R1), k = k2 (mod R2); then
+//   X[k] = sum_{n1,n2} x[n] w_R1^{n1 k1} w_R2^{n2 k2}
+// (w_R^{nk} = w_R1^{n1 k} w_R2^{n2 k}), so the R2 inner length-R1 DFTs and R1
+// outer length-R2 DFTs need no twiddles; both maps are compile-time.
+template <int R1, int R2>
+struct Crt {
+  // the k in [0, R1 R2) with k % R1 == k1 and k % R2 == k2
+  static constexpr int at(int k1, int k2) {
+    for (int k = 0; k < R1 * R2; ++k)
+      if (k % R1 == k1 && k % R2 == k2) return k;
+    return -1;
+  }
+};
+
+template <int R1, int R2, int SIGN, typename R>
+__device__ __forceinline__ void dft_pfa(cpx<R>* v) {
+  constexpr int RR = R1 * R2;
+  cpx<R> y[R2][R1];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    cpx<R> t[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) t[n1] = v[(R2 * n1 + R1 * n2) % RR];
+    dft<R1, SIGN>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) y[n2][k1] = t[k1];
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    cpx<R> t[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) t[n2] = y[n2][k1];
+    dft<R2, SIGN>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[Crt<R1, R2>::at(k1, k2)] = t[k2];
+  }
+}
+
+// Radix 9 = 3 x 3 by Cooley-Tukey: n = 3 n1 + n2, k = k1 + 3 k2,
+//   X[k] = sum_{n2} w_9^{n2 k1} w_3^{n2 k2} sum_{n1} x[3 n1 + n2] w_3^{n1 k1},
+// with the four non-trivial twiddles w_9^{m}, m = n2 k1 in {1, 2, 4}.
+template <int SIGN, typename R>
+__device__ __forceinline__ void dft9(cpx<R>* v) {
+  constexpr R c1 = R(0.7660444431189780352023927), s1 = R(0.6427876096865393263226434);
+  constexpr R c2 = R(0.1736481776669303488517166), s2 = R(0.9848077530122080593667430);
+  constexpr R c4 = R(-0.9396926207859083840541093), s4 = R(0.3420201433256687330440996);
+  cpx<R> y[3][3];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+    cpx<R> t[3] = {v[n2], v[3 + n2], v[6 + n2]};
+    dft<3, SIGN>(t);
+    y[n2][0] = t[0];
+    y[n2][1] = t[1];
+    y[n2][2] = t[2];
+  }
+  y[1][1] = crot<SIGN>(y[1][1], c1, s1);  // m = 1
+  y[1][2] = crot<SIGN>(y[1][2], c2, s2);  // m = 2
+  y[2][1] = crot<SIGN>(y[2][1], c2, s2);  // m = 2
+  y[2][2] = crot<SIGN>(y[2][2], c4, s4);  // m = 4
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    cpx<R> t[3] = {y[0][k1], y[1][k1], y[2][k1]};
+    dft<3, SIGN>(t);
+    v[k1] = t[0];
+    v[k1 + 3] = t[1];
+    v[k1 + 6] = t[2];
+  }
+}
+
+template <int RAD, int SIGN, typename R>
 __device__ __forceinline__ void dft(cpx<R>* v) {
   if constexpr (RAD == 2) dft2<SIGN>(v);
   else if constexpr (RAD == 4) dft4<SIGN>(v);
   else if constexpr (RAD == 8) dft8<SIGN>(v);
   else if constexpr (RAD == 16) dft16<SIGN>(v);
+  else if constexpr (RAD == 6) dft_pfa<2, 3, SIGN>(v);
+  else if constexpr (RAD == 10) dft_pfa<2, 5, SIGN>(v);
+  else if constexpr (RAD == 12) dft_pfa<4, 3, SIGN>(v);
+  else if constexpr (RAD == 9) dft9<SIGN>(v);
   else dft_odd<RAD, SIGN>(v);
 }
 

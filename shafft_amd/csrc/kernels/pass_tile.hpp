@@ -143,6 +143,17 @@ __device__ __forceinline__ void stage_last_regs(cpx<R> (&v)[EMAX], const cpx<R>*
   }
 }
 
+// The butterfly lane index as the stage loop sees it: an empty asm statement
+// marks it as redefined every stage, so the compiler computes each stage's
+// LDS addresses inside that stage's radix case instead of hoisting the
+// address sets of every radix case out of the stage loop (all of them live
+// at once: +80 VGPRs, and computed per tile for cases that never run).
+__device__ __forceinline__ int stage_lane(uint32_t tl) {
+  int v = (int)tl;
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // One Stockham stage of radix `rad` (wave-uniform) on this thread's line.
 // R16 = false leaves out the radix-16 butterfly (the largest register
 // footprint of all stages) for kernels whose plans use radix 8 at most.
@@ -157,8 +168,12 @@ __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t n
     case 3: stage_lds<R, 3, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 4: stage_lds<R, 4, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 5: stage_lds<R, 5, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 6: stage_lds<R, 6, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 7: stage_lds<R, 7, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 8: stage_lds<R, 8, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 9: stage_lds<R, 9, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 10: stage_lds<R, 10, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 12: stage_lds<R, 12, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 16:
       if constexpr (R16) stage_lds<R, 16, SIGN>(line, n, Ns, nsd, tws, tl, T);
       break;
@@ -436,15 +451,15 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   if constexpr (!BLUE) {
     const int s0 = reg_first ? 1 : 0;
     const int s1 = reg_last ? uni(a.nstages) - 1 : uni(a.nstages);
-    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX, R16>(a, s, line, (int)tl, lds, twb);
+    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX, R16>(a, s, line, stage_lane(tl), lds, twb);
     if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl, active, twb);
   } else {
     const int nst = uni(a.nstages);
-    for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0, R16>(a, s, line, (int)tl, lds, twb);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0, R16>(a, s, line, stage_lane(tl), lds, twb);
     const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
     for (int k = (int)tl; k < n; k += a.T) line[lp(k)] = cmul(line[lp(k)], h[k]);
     __syncthreads();
-    for (int s = 0; s < nst; ++s) stage_dispatch<R, +1, 0, R16>(a, s, line, (int)tl, lds, twb);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, +1, 0, R16>(a, s, line, stage_lane(tl), lds, twb);
   }
 
   // ---- store -----------------------------------------------------------------
