@@ -34,7 +34,6 @@ BatchPlan::BatchPlan(int count, const int* ndims, const int* dims_flat, int prec
     : prec_(prec), prog_(prec) {
   if (prec != 0 && prec != 1) fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
   if (count < 1 || !ndims || !dims_flat) fail(ERR_INVALID_DIM, "empty batch");
-  prog_.set_standalone_launch(false);  // passes run inside the batch kernel
   const int64_t cap = lds_cap(prec);
   int64_t off = 0;
   const int* d = dims_flat;

@@ -90,19 +90,12 @@ bool is_smooth7(int64_t n) {
   return n == 1;
 }
 
-std::vector<int> radix_plan(int64_t n, bool max8) {
+std::vector<int> radix_plan(int64_t n) {
   std::vector<int> r;
   int a = 0;
   while (n % 2 == 0) {
     n /= 2;
     ++a;
-  }
-  if (max8) {
-    // Radix-8 stages, the remainder as one radix 4 or 2 stage.
-    for (; a >= 3; a -= 3) r.push_back(8);
-    if (a == 2) r.push_back(4);
-    if (a == 1) r.push_back(2);
-    a = 0;
   }
   // Powers of two: radix-16 stages, the remainder as one radix 8/4/2 stage;
   // 2^(4q+1) is taken as 16^(q-1) * 8 * 4 rather than 16^q * 2.
@@ -461,11 +454,8 @@ void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, 
   // The fixed-length kernel (when eligible) dictates the radix sequence.
   const bool fixed_candidate = env_int("SHAFFT_AMD_FIXED", 1) && sa_fixed_supported((int)N) &&
                                blu_h == nullptr && !ld.active() && !st.active();
-  // The per-pass complex-float kernel runs radix 8 at most (its register
-  // cap); the batch kernel and complex double keep radix 16.
-  std::vector<int> rad = N > 1 ? (fixed_candidate ? fixed_radix_plan(N)
-                                                  : radix_plan(N, standalone_ && prec_ == 0))
-                               : std::vector<int>{};
+  std::vector<int> rad =
+      N > 1 ? (fixed_candidate ? fixed_radix_plan(N) : radix_plan(N)) : std::vector<int>{};
   if (rad.size() > SA_MAX_STAGES) fail(ERR_INTERNAL, "too many stages");
   std::vector<int> offs;
   p.tw = stage_table(rad, offs);
@@ -503,7 +493,6 @@ void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, 
   p.pre = ld.table;
   p.post = st.table;
   p.blu_h = blu_h;
-  p.scale = 1.0;
   configure_launch(p, prec_ ? 16 : 8, blu_h != nullptr);
   {
     int64_t grid = 0;

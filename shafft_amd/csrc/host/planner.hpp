@@ -90,15 +90,14 @@ int64_t lds_cap(int prec);
 // Longest line for a pass with strided elements (coalescing bound).
 int64_t lf_cap(int prec);
 bool is_smooth7(int64_t n);
-// radices for a 7-smooth n (product == n); max8: no radix-16 stage
-std::vector<int> radix_plan(int64_t n, bool max8 = false);
+// radices for a 7-smooth n (product == n)
+std::vector<int> radix_plan(int64_t n);
 
 class PassProgram {
  public:
   explicit PassProgram(int prec) : prec_(prec) {}
   // Passes run by the caller's own kernel (the batch launch) keep radix-16
   // stages; passes launched one by one use the per-pass kernels' plans.
-  void set_standalone_launch(bool on) { standalone_ = on; }
   // Append the passes of a length-N DFT over the lines `lines`
   // (outermost first), input element stride iks, output element stride oks.
   void add_transform(int64_t N, const std::vector<Digit>& lines, int64_t iks, int64_t oks);
@@ -138,7 +137,6 @@ class PassProgram {
   std::vector<PassSpec> passes_;
   void add_transform_impl(int64_t N, const std::vector<Digit>& lines, int64_t iks, int64_t oks);
   bool lf_split_ = true;  // split strided lines at lf_cap (else only at lds_cap)
-  bool standalone_ = true;  // passes launched one by one by the per-pass kernels
   std::vector<DevBuf> tables_;
   std::map<std::vector<int>, std::pair<const void*, std::vector<int>>> stage_cache_;
 

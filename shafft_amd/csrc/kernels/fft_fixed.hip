@@ -2,7 +2,7 @@
 // compile time (32 .. 8192), the hot shapes of the slab path.
 //
 // Same pass semantics as fft_pass.hip (pass_desc.h addressing, four-step
-// twiddle, scale), specialised so that
+// twiddle), specialised so that
 //   * every Stockham index is a compile-time shift/or, and LDS addresses
 //     are one base register plus immediate offsets;
 //   * each thread owns E elements (k = tl + T*m, T = N/E threads per line;
@@ -184,12 +184,11 @@ __device__ __forceinline__ StoreCtx<R> store_ctx(const sa_pass_t& a, uint32_t b,
   return c;
 }
 
-// Store one output x of slot m with its twiddle w and the scale.
+// Store one output x of slot m with its four-step twiddle w.
 template <typename R, int SIGN>
 __device__ __forceinline__ void put(const sa_pass_t& a, const StoreCtx<R>& c, int m, cpx<double> w,
                                     cpx<R> x) {
   if (a.ptw_level >= 0) x = cmul_dir<SIGN>(x, cpx<R>{(R)w.x, (R)w.y});
-  if (a.scale != 1.0) x = cscale(x, (R)a.scale);
   st(c.dst + m * c.step, x, a.nt != 0);
 }
 

@@ -20,25 +20,24 @@
 
 namespace sa {
 
-// Register budget and workgroup size.  The complex-float kernel runs plans
-// of radix 8 at most (the radix-16 butterfly sets the whole kernel's
-// register footprint: ~155 VGPRs without it, 3 waves per SIMD); the
-// complex-double kernel keeps radix 16.  Occupancy comes from workgroups of
-// at most 256 threads (the planner's tile choice): several independent tiles
-// per CU overlap one's HBM loads with another's LDS stages.  Measured over
-// the 30 largest jobs of each precision of the serving stream: complex64
-// 2.18 -> 2.70 TB/s, complex128 3.31 -> 3.76 TB/s against 512-thread tiles; a
-// 128-VGPR cap (4 waves per SIMD, with spills) lost to it at every tile size.
+// Register budget and occupancy.  With the stage loop's LDS addresses
+// computed inside each stage (stage_lane), the LDS-mode kernels need ~75
+// (complex float) / ~127 (complex double) VGPRs, 6 / 4 waves per SIMD, for
+// every radix up to 16.  The planner's tiles hold at most 64 KiB of data
+// (512 threads complex float, 256 complex double), so several independent
+// tiles share a CU and one's HBM loads overlap another's LDS stages; the
+// stages themselves are VALU-issue bound (~120 VALU instructions per element
+// over 3 stages, load and store, by SQ counters).
 template <int SIGN, int MODE, bool TWL>
 __global__ __launch_bounds__(512) void fft_lds_pass_f(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<float, SIGN, MODE, 0, false, TWL>(a, blockIdx.x, reinterpret_cast<cpx<float>*>(lds_raw));
+  pass_tile<float, SIGN, MODE, 0, TWL>(a, blockIdx.x, reinterpret_cast<cpx<float>*>(lds_raw));
 }
 
 template <int SIGN, int MODE, bool TWL>
 __global__ __launch_bounds__(512) void fft_lds_pass_d(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<double, SIGN, MODE, 0, true, TWL>(a, blockIdx.x, reinterpret_cast<cpx<double>*>(lds_raw));
+  pass_tile<double, SIGN, MODE, 0, TWL>(a, blockIdx.x, reinterpret_cast<cpx<double>*>(lds_raw));
 }
 
 template <typename R, int SIGN, int MODE, bool TWL>

@@ -83,8 +83,6 @@ typedef struct {
    * (FFT of the conjugate chirp, already divided by n), then runs backward
    * stages, all inside LDS.  The direction is carried by pre/post.       */
   const void* blu_h;
-
-  double scale; /* applied on the store when != 1                        */
 } sa_pass_t;
 
 /* Launch one pass.  prec: 0 = complex float, 1 = complex double.
@@ -92,7 +90,7 @@ typedef struct {
 int sa_launch_pass(const sa_pass_t* p, int prec, int sign, void* stream);
 
 /* Fixed-length persistent kernel (power-of-two n in [32, 8192], no load or
- * store hooks other than the four-step twiddle and scale).  `grid` is the
+ * store hooks other than the four-step twiddle).  `grid` is the
  * number of persistent workgroups. */
 int sa_fixed_supported(int n);
 int sa_fixed_elems(int n, int prec); /* elements per thread (n / threads per line) */

@@ -155,14 +155,9 @@ __device__ __forceinline__ int stage_lane(uint32_t tl) {
 }
 
 // One Stockham stage of radix `rad` (wave-uniform) on this thread's line.
-// R16 = false leaves out the radix-16 butterfly (the largest register
-// footprint of all stages) for kernels whose plans use radix 8 at most.
-template <typename R, int SIGN, bool R16 = true>
+template <typename R, int SIGN>
 __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t nsd,
                                           const cpx<R>* tws, cpx<R>* line, int tl, int T) {
-  if constexpr (!R16) {
-    if (rad == 16) __builtin_trap();  // the planner never emits it here
-  }
   switch (rad) {
     case 2: stage_lds<R, 2, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 3: stage_lds<R, 3, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
@@ -175,7 +170,7 @@ __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t n
     case 10: stage_lds<R, 10, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 12: stage_lds<R, 12, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 16:
-      if constexpr (R16) stage_lds<R, 16, SIGN>(line, n, Ns, nsd, tws, tl, T);
+      stage_lds<R, 16, SIGN>(line, n, Ns, nsd, tws, tl, T);
       break;
     default: break;
   }
@@ -184,7 +179,7 @@ __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t n
 // MX != 0 (complex float only): radix-8/16 stages on the matrix cores in
 // split precision (mfma_stage.hpp); `lds` is the tile's LDS base.
 // twb: the stage twiddle table (global, or staged in LDS by the caller).
-template <typename R, int SIGN, int MX = 0, bool R16 = true>
+template <typename R, int SIGN, int MX = 0>
 __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>* line, int tl,
                                                cpx<R>* lds, const cpx<R>* twb) {
   const cpx<R>* tws = twb + a.twoff[s];
@@ -200,7 +195,7 @@ __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>
       return;
     }
   }
-  stage_run<R, SIGN, R16>(uni(a.radix[s]), a.n, a.ns[s], a.nsdiv[s], tws, line, tl, a.T);
+  stage_run<R, SIGN>(uni(a.radix[s]), a.n, a.ns[s], a.nsdiv[s], tws, line, tl, a.T);
 }
 
 template <typename R, int SIGN>
@@ -321,8 +316,6 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
   cpx<R>* __restrict__ dst = reinterpret_cast<cpx<R>*>(a.out) + o.out;
   const cpx<R>* __restrict__ post = reinterpret_cast<const cpx<R>*>(a.post);
   const bool nt = a.nt != 0;
-  const bool do_scale = a.scale != 1.0;
-  const R sc = (R)a.scale;
   cpx<R>* __restrict__ q0 = dst + (int64_t)k0 * a.oks;  // element k0 + i*T at q0 + i*ostep
   const int64_t ostep = (int64_t)T * a.oks;
   if (a.ptw_level >= 0) {
@@ -346,7 +339,6 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
         cpx<R> x = cmul_dir<SIGN>(v[i], cpx<R>{(R)w.x, (R)w.y});
-        if (do_scale) x = cscale(x, sc);
         gstore(q0 + i * ostep, x, nt);
       }
       if (i + 1 < EMAX) w = cmul(w, step);
@@ -357,7 +349,6 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
         cpx<R> x = v[i];
-        if (do_scale) x = cscale(x, sc);
         gstore(q0 + i * ostep, x, nt);
       }
     }
@@ -372,8 +363,7 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
         if (m < a.st_limit) {
           cpx<R> x = v[i];
           if (post) x = cmul(x, post[m]);
-          if (do_scale) x = cscale(x, sc);
-          gstore(q0 + i * ostep, x, nt);
+            gstore(q0 + i * ostep, x, nt);
         }
       }
     }
@@ -398,7 +388,7 @@ __device__ __forceinline__ const cpx<R>* stage_twiddles_to_lds(const sa_pass_t& 
 // One tile (C consecutive lines) of a pass, run by the whole workgroup.
 // Threads t >= C*T of a larger workgroup idle through the barriers.
 // TWL: the stages read their twiddles from LDS (a.tw_lds > 0, not Bluestein).
-template <typename R, int SIGN, int MODE, int MX = 0, bool R16 = true, bool TWL = false>
+template <typename R, int SIGN, int MODE, int MX = 0, bool TWL = false>
 __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx<R>* lds) {
   constexpr bool BLUE = MODE == MODE_BLUE;
 
@@ -459,15 +449,15 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   if constexpr (!BLUE) {
     const int s0 = reg_first ? 1 : 0;
     const int s1 = reg_last ? uni(a.nstages) - 1 : uni(a.nstages);
-    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX, R16>(a, s, line, stage_lane(tl), lds, twb);
+    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX>(a, s, line, stage_lane(tl), lds, twb);
     if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl, active, twb);
   } else {
     const int nst = uni(a.nstages);
-    for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0, R16>(a, s, line, stage_lane(tl), lds, twb);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0>(a, s, line, stage_lane(tl), lds, twb);
     const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
     for (int k = (int)tl; k < n; k += a.T) line[lp(k)] = cmul(line[lp(k)], h[k]);
     __syncthreads();
-    for (int s = 0; s < nst; ++s) stage_dispatch<R, +1, 0, R16>(a, s, line, stage_lane(tl), lds, twb);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, +1, 0>(a, s, line, stage_lane(tl), lds, twb);
   }
 
   // ---- store -----------------------------------------------------------------

@@ -210,7 +210,6 @@ void run_pass(const sa_pass_t& p, int sign) {
             v *= post[m];
           }
         }
-        if (p.scale != 1.0) v *= (R)p.scale;
         chk("out", outp, ooff[b - b0] + (int64_t)k * p.oks, sizeof(C<R>));
         outp[ooff[b - b0] + (int64_t)k * p.oks] = v;
       }
