@@ -277,28 +277,23 @@ __device__ __forceinline__ void load_line(const sa_pass_t& a, const LineOff& o, 
   const cpx<R>* __restrict__ src = reinterpret_cast<const cpx<R>*>(a.in) + o.in;
   const bool nt = a.nt != 0;
   const bool plain = a.pre == nullptr && a.ld_level < 0 && a.ld_kmul == 1 && a.ld_limit >= n;
-  // Element k0 + i*T sits at p0 + i*step: one 64-bit add per element with a
-  // wave-uniform step, not a 64-bit multiply.
-  const cpx<R>* __restrict__ p0 = src + (int64_t)k0 * a.iks;
-  const int64_t step = (int64_t)T * a.iks;
   if (plain) {
 #pragma unroll
     for (int i = 0; i < EMAX; ++i) {
       const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) v[i] = gload(p0 + i * step, nt);
+      if (k < (uint32_t)n) v[i] = gload(src + (int64_t)k * a.iks, nt);
     }
   } else {
     const cpx<R>* __restrict__ pre = reinterpret_cast<const cpx<R>*>(a.pre);
-    const int64_t m0 = (int64_t)k0 * a.ld_kmul + o.ld_base;
-    const int64_t mstep = (int64_t)T * a.ld_kmul;
+    const int64_t base = o.ld_base;
 #pragma unroll
     for (int i = 0; i < EMAX; ++i) {
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
-        const int64_t m = m0 + i * mstep;
+        const int64_t m = (int64_t)k * a.ld_kmul + base;
         cpx<R> x = {R(0), R(0)};
         if (m < a.ld_limit) {
-          x = gload(p0 + i * step, nt);
+          x = gload(src + (int64_t)k * a.iks, nt);
           if (pre) x = cmul(x, pre[m]);
         }
         v[i] = x;
@@ -316,8 +311,6 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
   cpx<R>* __restrict__ dst = reinterpret_cast<cpx<R>*>(a.out) + o.out;
   const cpx<R>* __restrict__ post = reinterpret_cast<const cpx<R>*>(a.post);
   const bool nt = a.nt != 0;
-  cpx<R>* __restrict__ q0 = dst + (int64_t)k0 * a.oks;  // element k0 + i*T at q0 + i*ostep
-  const int64_t ostep = (int64_t)T * a.oks;
   if (a.ptw_level >= 0) {
     // W_N^{(digit * k) mod N} for k = k0 + i*T is geometric in i: look up
     // W^{e(k0)} and the step W^{digit*T} once (two-level table) and form the
@@ -339,7 +332,7 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
         cpx<R> x = cmul_dir<SIGN>(v[i], cpx<R>{(R)w.x, (R)w.y});
-        gstore(q0 + i * ostep, x, nt);
+        gstore(dst + (int64_t)k * a.oks, x, nt);
       }
       if (i + 1 < EMAX) w = cmul(w, step);
     }
@@ -349,21 +342,20 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
         cpx<R> x = v[i];
-        gstore(q0 + i * ostep, x, nt);
+        gstore(dst + (int64_t)k * a.oks, x, nt);
       }
     }
   } else {
-    const int64_t m0 = (int64_t)k0 * a.st_kmul + o.st_base;
-    const int64_t mstep = (int64_t)T * a.st_kmul;
+    const int64_t base = o.st_base;
 #pragma unroll
     for (int i = 0; i < EMAX; ++i) {
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
-        const int64_t m = m0 + i * mstep;
+        const int64_t m = (int64_t)k * a.st_kmul + base;
         if (m < a.st_limit) {
           cpx<R> x = v[i];
           if (post) x = cmul(x, post[m]);
-            gstore(q0 + i * ostep, x, nt);
+            gstore(dst + (int64_t)k * a.oks, x, nt);
         }
       }
     }
