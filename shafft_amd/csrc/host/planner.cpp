@@ -305,8 +305,14 @@ static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
     tw_elems = std::max<int64_t>(tw_elems, (int64_t)p.twoff[s] + (int64_t)p.ns[s] * (p.radix[s] - 1));
   p.tw_lds = 0;
   if (!bluestein && env_int("SHAFFT_AMD_GTW_LDS", 1) && tw_elems > 0 &&
-      ((int64_t)C * stride + tw_elems) * elem_bytes <= budget)
+      ((int64_t)C * stride + tw_elems) * elem_bytes <= budget) {
+    // ... unless the table costs workgroups per CU: long single-line tiles
+    // (n ~ 6000, a table of ~n elements) would drop from 2 to 1 per CU,
+    // 2.9 -> 1.6 TB/s.
     p.tw_lds = (int32_t)tw_elems;
+    const int prec = elem_bytes == 16 ? 1 : 0;
+    if (sa_pass_occupancy(&p, prec, 1) < sa_pass_occupancy(&p, prec, 0)) p.tw_lds = 0;
+  }
 }
 
 static int device_cus() {

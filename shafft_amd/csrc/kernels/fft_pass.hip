@@ -99,6 +99,29 @@ static hipError_t launch_prec(const sa_pass_t* p, int sign, hipStream_t st) {
 
 extern "C" int sa_max_lds_bytes(void) { return 160 * 1024; }
 
+namespace sa {
+template <typename R>
+static const void* pass_kernel(const sa_pass_t* p, bool twl) {
+  if (p->blu_h) return PassKernel<R, -1, MODE_BLUE, false>::fn();
+  if (p->reg_first && p->reg_last)
+    return twl ? PassKernel<R, -1, MODE_REGS, true>::fn() : PassKernel<R, -1, MODE_REGS, false>::fn();
+  return twl ? PassKernel<R, -1, MODE_LDS, true>::fn() : PassKernel<R, -1, MODE_LDS, false>::fn();
+}
+}  // namespace sa
+
+extern "C" int sa_pass_occupancy(const sa_pass_t* p, int prec, int twl) {
+  const size_t esz = prec ? 16 : 8;
+  const size_t lds = ((size_t)p->C * p->ldsstride + (twl ? (size_t)p->tw_lds : 0)) * esz;
+  const void* f = prec ? sa::pass_kernel<double>(p, twl != 0) : sa::pass_kernel<float>(p, twl != 0);
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, sa_max_lds_bytes());
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, p->C * p->T, lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return blocks;
+}
+
 extern "C" int sa_launch_pass(const sa_pass_t* p, int prec, int sign, void* stream) {
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   return prec == 0 ? (int)sa::launch_prec<float>(p, sign, st) : (int)sa::launch_prec<double>(p, sign, st);

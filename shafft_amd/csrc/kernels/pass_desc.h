@@ -88,6 +88,9 @@ typedef struct {
 /* Launch one pass.  prec: 0 = complex float, 1 = complex double.
  * sign: -1 forward, +1 backward.  Returns a hipError_t value. */
 int sa_launch_pass(const sa_pass_t* p, int prec, int sign, void* stream);
+/* Workgroups of the pass kernel per CU for p's launch shape, with (twl = 1)
+ * or without its stage twiddles staged in LDS; 0 if it cannot launch. */
+int sa_pass_occupancy(const sa_pass_t* p, int prec, int twl);
 
 /* Fixed-length persistent kernel (power-of-two n in [32, 8192], no load or
  * store hooks other than the four-step twiddle).  `grid` is the

@@ -235,6 +235,11 @@ extern "C" int sa_fixed_supported(int n) {
 }
 
 extern "C" int sa_fixed_occupancy(int, int, int, int, size_t) { return 1; }
+// LDS-limited workgroups per CU (the registers are not modelled here).
+extern "C" int sa_pass_occupancy(const sa_pass_t* p, int prec, int twl) {
+  const int64_t lds = ((int64_t)p->C * p->ldsstride + (twl ? p->tw_lds : 0)) * (prec ? 16 : 8);
+  return lds > 160 * 1024 ? 0 : (int)((160 * 1024) / std::max<int64_t>(lds, 1));
+}
 extern "C" int sa_fixed_elems(int n, int prec) { return (prec == 0 && (n == 512 || n == 1024)) ? 32 : 16; }
 extern "C" int sa_fixed_max_threads(int) { return 512; }
 
