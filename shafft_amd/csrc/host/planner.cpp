@@ -499,6 +499,7 @@ void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, 
   p.pre = ld.table;
   p.post = st.table;
   p.blu_h = blu_h;
+  p.blu_level = -1;
   configure_launch(p, prec_ ? 16 : 8, blu_h != nullptr);
   {
     int64_t grid = 0;
@@ -711,6 +712,10 @@ void PassProgram::add_transform_impl(int64_t N, const std::vector<Digit>& lines,
     ps.blu_t[1] = th[1];
     return;
   }
+  if (iks == 1 && oks == 1 && env_int("SHAFFT_AMD_BLUE3", 1) &&
+      bluestein_three_pass(N, M, lines, tc, H)) {
+    return;
+  }
   // Long Bluestein: forward M-FFT (chirp + zero padding on its first load,
   // filter product on its last store), then backward M-FFT (output chirp and
   // truncation on its last store).  The intermediate is [lines][M][u] as
@@ -745,6 +750,92 @@ void PassProgram::add_transform_impl(int64_t N, const std::vector<Digit>& lines,
       ps.blu_t[0] = ps.blu_t[1] = nullptr;
     }
   }
+}
+
+// Bluestein convolution of length M = N1 * N2 in three passes over a
+// contiguous axis (iks = oks = 1), keeping the spectrum in transposed order:
+//   A: N1-point forward DFTs along n1 (element stride N2) of the chirped,
+//      zero-padded input, times W_M^{n2 k1}, stored as Y[k1][n2];
+//   B: per row k1, the N2-point forward DFT over n2, times the filter
+//      H[k1 + N1 k2] (a transposed copy, one row per k1), the N2-point
+//      inverse DFT over k2, times conj(W_M^{n2 k1}) -- all in LDS;
+//   C: N1-point inverse DFTs along k1, times the output chirp, stored for
+//      n = N2 n1 + n2 < N.
+// The inverse four-step x[N2 n1 + n2] = sum_k1 w_N1^{-n1 k1} w_M^{-n2 k1}
+// sum_k2 X[k1 + N1 k2] w_N2^{-n2 k2} needs X only row by row, so the
+// forward transform's final transpose and the inverse's first one cancel:
+// 3 passes over M instead of the 6 of two separate M-point transforms.
+bool PassProgram::bluestein_three_pass(int64_t N, int64_t M, const std::vector<Digit>& lines,
+                                       const void* const tc[2],
+                                       const std::vector<std::complex<long double>> H[2]) {
+  const int64_t lfc = lf_cap(prec_);
+  int64_t N2 = std::min<int64_t>(4096, M / 16);
+  int64_t N1 = M / N2;
+  if (N1 > lfc) {
+    N1 = lfc;
+    N2 = M / N1;
+  }
+  if (N1 < 2 || N2 < 2 || N1 * N2 != M || N1 > lfc || N2 > std::min<int64_t>(8192, lds_cap(prec_)))
+    return false;
+  int64_t U = 1;
+  int ins = 0;
+  const std::vector<int64_t> cs = inter_strides(lines, M, &U, &ins);
+  if (U != 1) return false;
+  const int64_t L = prod_cnt(lines);
+  std::vector<cld> Ht[2];
+  for (int d = 0; d < 2; ++d) {
+    Ht[d].resize(M);
+    for (int64_t k1 = 0; k1 < N1; ++k1)
+      for (int64_t k2 = 0; k2 < N2; ++k2) Ht[d][k1 * N2 + k2] = H[d][k1 + N1 * k2];
+  }
+  const void* th[2] = {upload(Ht[0]), upload(Ht[1])};
+  const size_t first = passes_.size();
+
+  std::vector<Digit> la = lines;
+  for (size_t j = 0; j < la.size(); ++j) la[j].os = cs[j];
+  la.push_back(Digit{N2, 1, 1});
+  const int n2a = (int)la.size() - 1;
+  ElemHook lda;
+  lda.table = tc[0];
+  lda.limit = N;
+  lda.kmul = N2;
+  lda.level = n2a;
+  lda.dmul = 1;
+  emit_single(N1, la, N2, N2, -1, lda, ElemHook{}, nullptr, n2a, M, L * M);
+
+  std::vector<Digit> lb = lines;
+  for (size_t j = 0; j < lb.size(); ++j) lb[j].is = lb[j].os = cs[j];
+  lb.push_back(Digit{N1, N2, N2});
+  const int k1b = (int)lb.size() - 1;
+  emit_single(N2, lb, 1, 1, -1, ElemHook{}, ElemHook{}, th[0], k1b, M, L * M);
+  {
+    sa_pass_t& p = passes_.back().p;
+    p.ptw_conj = 1;
+    p.blu_level = p.ptw_level;  // the k1 digit, after emit_single's remapping
+    p.blu_hstride = N2;
+  }
+
+  std::vector<Digit> lc = lines;
+  for (size_t j = 0; j < lc.size(); ++j) lc[j].is = cs[j];
+  lc.push_back(Digit{N2, 1, 1});
+  const int n2c = (int)lc.size() - 1;
+  ElemHook stc;
+  stc.table = tc[0];
+  stc.limit = N;
+  stc.kmul = N2;
+  stc.level = n2c;
+  stc.dmul = 1;
+  emit_single(N1, lc, N2, N2, +1, ElemHook{}, stc, nullptr, -1, 0, L * N);
+
+  PassSpec* ps = &passes_[first];
+  for (int d = 0; d < 3; ++d) ps[d].dir_tables = true;
+  ps[0].pre_t[0] = tc[0];
+  ps[0].pre_t[1] = tc[1];
+  ps[1].blu_t[0] = th[0];
+  ps[1].blu_t[1] = th[1];
+  ps[2].post_t[0] = tc[0];
+  ps[2].post_t[1] = tc[1];
+  return true;
 }
 
 PassProgram::SlotMap PassProgram::assign(Mode mode, int64_t data_elems) const {

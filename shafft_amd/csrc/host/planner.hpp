@@ -96,8 +96,6 @@ std::vector<int> radix_plan(int64_t n);
 class PassProgram {
  public:
   explicit PassProgram(int prec) : prec_(prec) {}
-  // Passes run by the caller's own kernel (the batch launch) keep radix-16
-  // stages; passes launched one by one use the per-pass kernels' plans.
   // Append the passes of a length-N DFT over the lines `lines`
   // (outermost first), input element stride iks, output element stride oks.
   void add_transform(int64_t N, const std::vector<Digit>& lines, int64_t iks, int64_t oks);
@@ -136,6 +134,9 @@ class PassProgram {
   int prec_;
   std::vector<PassSpec> passes_;
   void add_transform_impl(int64_t N, const std::vector<Digit>& lines, int64_t iks, int64_t oks);
+  bool bluestein_three_pass(int64_t N, int64_t M, const std::vector<Digit>& lines,
+                            const void* const tc[2],
+                            const std::vector<std::complex<long double>> H[2]);
   bool lf_split_ = true;  // split strided lines at lf_cap (else only at lds_cap)
   std::vector<DevBuf> tables_;
   std::map<std::vector<int>, std::pair<const void*, std::vector<int>>> stage_cache_;

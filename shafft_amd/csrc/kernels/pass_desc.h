@@ -62,6 +62,7 @@ typedef struct {
    * with digit = (b0, b1, b2)[ptw_level]; W from a two-level table
    * W^e = hi[e >> ptw_shift] * lo[e & ((1 << ptw_shift) - 1)].           */
   int32_t ptw_level; /* -1: none */
+  int32_t ptw_conj;  /* 1: multiply by conj(W) instead                   */
   int32_t ptw_shift;
   int64_t ptw_N;
   const void* ptw_lo;
@@ -83,6 +84,10 @@ typedef struct {
    * (FFT of the conjugate chirp, already divided by n), then runs backward
    * stages, all inside LDS.  The direction is carried by pre/post.       */
   const void* blu_h;
+  /* Per-line filter: line b multiplies by blu_h + digit[blu_level] *
+   * blu_hstride (blu_level < 0: the same filter for every line).         */
+  int32_t blu_level;
+  int64_t blu_hstride;
 } sa_pass_t;
 
 /* Launch one pass.  prec: 0 = complex float, 1 = complex double.

@@ -327,11 +327,12 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
     };
     cpx<double> w = w64((dig * k0) % N);
     const cpx<double> step = w64((dig * T) % N);
+    const R cs = a.ptw_conj ? R(-1) : R(1);  // conj(W) on request
 #pragma unroll
     for (int i = 0; i < EMAX; ++i) {
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
-        cpx<R> x = cmul_dir<SIGN>(v[i], cpx<R>{(R)w.x, (R)w.y});
+        cpx<R> x = cmul_dir<SIGN>(v[i], cpx<R>{(R)w.x, cs * (R)w.y});
         gstore(dst + (int64_t)k * a.oks, x, nt);
       }
       if (i + 1 < EMAX) w = cmul(w, step);
@@ -447,6 +448,16 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
     const int nst = uni(a.nstages);
     for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0>(a, s, line, stage_lane(tl), lds, twb);
     const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
+    if (a.blu_level >= 0 && active) {
+      const uint32_t b = line0 + c;
+      if (b < (uint32_t)a.batch) {
+        const uint32_t q2 = fdiv_d(b, a.n2div);
+        const uint32_t d2 = b - q2 * (uint32_t)a.n2;
+        const uint32_t d0 = fdiv_d(q2, a.n1div);
+        const uint32_t d1 = q2 - d0 * (uint32_t)a.n1;
+        h += (int64_t)sel3(a.blu_level, d0, d1, d2) * a.blu_hstride;
+      }
+    }
     for (int k = (int)tl; k < n; k += a.T) line[lp(k)] = cmul(line[lp(k)], h[k]);
     __syncthreads();
     for (int s = 0; s < nst; ++s) stage_dispatch<R, +1, 0>(a, s, line, stage_lane(tl), lds, twb);

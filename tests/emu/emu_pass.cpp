@@ -180,9 +180,10 @@ void run_pass(const sa_pass_t& p, int sign) {
       } else {
         stages<R>(p, d, -1);
         const C<R>* h = reinterpret_cast<const C<R>*>(p.blu_h);
+        const int64_t hoff = p.blu_level >= 0 ? dg[p.blu_level] * p.blu_hstride : 0;
         for (int k = 0; k < n; ++k) {
-          chk("blu_h", h, k, sizeof(C<R>));
-          d[k] *= h[k];
+          chk("blu_h", h, hoff + k, sizeof(C<R>));
+          d[k] *= h[hoff + k];
         }
         stages<R>(p, d, +1);
       }
@@ -200,7 +201,7 @@ void run_pass(const sa_pass_t& p, int sign) {
           chk("ptw_hi", hi, (int64_t)(e >> p.ptw_shift), sizeof(C<R>));
           chk("ptw_lo", lo, (int64_t)(e & ((1ull << p.ptw_shift) - 1)), sizeof(C<R>));
           const C<R> w = hi[e >> p.ptw_shift] * lo[e & ((1ull << p.ptw_shift) - 1)];
-          v *= conj_if(w, sign);
+          v *= conj_if(p.ptw_conj ? std::conj(w) : w, sign);
         } else {
           const int64_t m = (int64_t)k * p.st_kmul + (p.st_level >= 0 ? dg[p.st_level] * p.st_dmul : 0) +
                             (p.st_level2 >= 0 ? dg[p.st_level2] * p.st_dmul2 : 0);

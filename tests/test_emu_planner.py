@@ -84,7 +84,29 @@ for n, prec in CASES:
 
 def test_1d_plans_hardware_cap(emu_lib):
     out = _run_in_subprocess(SUB.replace("CASES", "[(1,1),(7,0),(4096,1),(8192,0),(16384,1),(100000,1),(40009,1)]"), {})
-    assert "40009 1 1 4" in out  # long Bluestein: 4 passes
+    assert "40009 1 1 3" in out  # long Bluestein: 3 passes (transposed spectrum)
+
+
+def test_long_bluestein_three_pass(emu_lib):
+    """Prime lengths past the LDS cap on a contiguous axis run Bluestein's
+    convolution in 3 passes (forward N1-point columns with the chirp and
+    W_M twiddle; per-row N2-point forward DFT, transposed filter, inverse DFT
+    and conjugate twiddle; inverse N1-point columns with the output chirp),
+    with other axes as outer lines; both directions, both precisions, against
+    the direct DFT."""
+    L = emu_lib
+    for dims, axes, prec, want in [((2, 4099), [1], 1, 3), ((9001,), [0], 0, 3),
+                                   ((3, 2, 4099), [1, 2], 1, 4)]:
+        x = oracle.uniform_input(dims, 12, DT[prec])
+        p = api.NDBackendPlan(list(dims), axes, prec, L=L)
+        assert p.num_passes() == want, p.describe()
+        a, b = api.DeviceBuffer(x.size, prec, L), api.DeviceBuffer(x.size, prec, L)
+        for d, sign in ((api.FORWARD, -1), (api.BACKWARD, 1)):
+            a.upload(x.ravel())
+            dd, _ = p.execute(a.ptr, b.ptr, d)
+            y = api.download_ptr(dd, x.size, prec, L).reshape(dims)
+            ref = oracle.dftn(x.astype(np.complex128), axes, sign)
+            assert oracle.check_rel_error(y, ref, TOL[prec]), (dims, d, oracle.rel_l2(y, ref))
 
 
 def test_1d_plans_small_cap_multilevel(emu_lib):
@@ -399,7 +421,7 @@ def test_single_pass_extents(emu_lib):
     n = 40009
     p = api.OneDBackendPlan(n, 1, L=L)
     ext = [p.pass_extents(i) for i in range(p.num_passes())]
-    assert p.num_passes() == 4 and ext[0] == (n, 131072) and ext[-1] == (131072, n)
+    assert p.num_passes() == 3 and ext[0] == (n, 131072) and ext[-1] == (131072, n)
     big = max(max(e) for e in ext)
     a, b = api.DeviceBuffer(big, 1, L), api.DeviceBuffer(big, 1, L)
     a.upload(oracle.uniform_input((big,), 3))
