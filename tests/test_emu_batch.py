@@ -49,6 +49,18 @@ def test_batch_backward_inplace_and_direct_launch(emu_lib):
             assert oracle.rel_l2(got, oracle.dftn(x, None, +1)) < 1e-13
 
 
+def test_batch_prime_axes_both_directions(emu_lib):
+    """Prime axes inside a batch run the in-LDS Bluestein pass of the batch
+    kernel with its direction-dependent chirp and filter tables.  (Axes
+    longer than one LDS pass are refused by the batch planner.)"""
+    shapes = [(1009,), (2, 1031), (8, 12)]
+    for direction, sign in ((0, -1), (1, +1)):
+        bp, xs, y = _run(emu_lib, shapes, 1, direction=direction)
+        for i, (sh, x) in enumerate(zip(shapes, xs)):
+            got = y[bp.offsets[i]:bp.offsets[i] + x.size].reshape(sh)
+            assert oracle.check_rel_error(got, oracle.dftn(x, None, sign), 1e-12), (sh, direction)
+
+
 def test_batch_graph_replays_match(emu_lib):
     L = emu_lib
     shapes = draw_shapes(6, 3, lo=4, hi=32)
