@@ -47,6 +47,20 @@ def test_partial_axes_reference_cases(gpu_lib):
             assert err < 1e-10
 
 
+@pytest.mark.parametrize("dims,axes,prec", [((3, 4099), (1,), api.Z2Z), ((2, 9001), (1,), api.C2C),
+                                             ((3, 2, 4099), (1, 2), api.Z2Z)])
+def test_long_prime_axis_three_pass_bluestein(gpu_lib, dims, axes, prec):
+    """A prime contiguous axis past the LDS cap (three-pass Bluestein with
+    other axes as outer lines), both directions, against the oracle."""
+    x = oracle.uniform_input(dims, 17, DT[prec])
+    for direction, sign in ((api.FORWARD, -1), (api.BACKWARD, +1)):
+        y = run_seam(list(dims), list(axes), prec, x, direction)
+        ref = oracle.dftn(x.astype(np.complex128), axes, sign)
+        e = oracle.rel_l2(y, ref)
+        print(f"{dims} axes {axes} prec {prec} dir {direction}: relL2 {e:.3e}", flush=True)
+        assert oracle.check_rel_error(y, ref, TOL[prec])
+
+
 HIGHDIM_CASES = [(3, 2, 4, 3, 2), (2, 3, 2, 3, 2, 3), (3, 2, 3, 2, 3, 2, 3), (2, 3, 2, 3, 2, 3, 2, 3),
                  (3, 2, 3, 2, 3, 2, 3, 2, 3), (2, 3) * 5, (2, 3) * 5 + (2,),
                  (2, 2, 2, 3, 2, 2, 2, 3, 2, 2, 2, 3)]
