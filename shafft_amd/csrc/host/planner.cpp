@@ -90,7 +90,9 @@ bool is_smooth7(int64_t n) {
   return n == 1;
 }
 
-std::vector<int> radix_plan(int64_t n) {
+static int env_int(const char* name, int dflt);
+
+static std::vector<int> radix_plan_greedy(int64_t n) {
   std::vector<int> r;
   int a = 0;
   while (n % 2 == 0) {
@@ -142,6 +144,81 @@ std::vector<int> radix_plan(int64_t n) {
   for (int i = 0; i < e7; ++i) r.push_back(7);
   for (int i = 0; i < e5; ++i) r.push_back(5);
   if (e3) r.push_back(3);
+  return r;
+}
+
+// Radix sequence with the fewest, cheapest stages.  Every LDS stage costs
+// about the same per element (~20-30 VALU instructions: addresses, twiddle,
+// butterfly share, barrier), so the plan minimises an estimate of
+// sum(stage cost) over factorisations of n into the kernel's butterflies:
+//   cost(R) = (butterfly ops + 4 (R-1) twiddle + 8 R address + 10) / R
+//             x rounds factor (per-line numbering runs ceil(16/R) rounds for
+//               16/R butterflies per thread; radices above 16 are numbered
+//               densely over the tile, one round).
+// Powers of two go first (largest first), then the rest, largest first.
+std::vector<int> radix_plan(int64_t n, int maxrad) {
+  if (!env_int("SHAFFT_AMD_WIDE", 1)) return radix_plan_greedy(n);
+  int e[4] = {0, 0, 0, 0};
+  int64_t m = n;
+  const int pr[4] = {2, 3, 5, 7};
+  for (int i = 0; i < 4; ++i)
+    for (; m % pr[i] == 0; m /= pr[i]) ++e[i];
+  if (m != 1) fail(ERR_INTERNAL, "radix_plan: length is not 7-smooth");
+  struct Rad {
+    int r, e2, e3, e5, e7;
+    double ops;  // butterfly arithmetic, instructions per butterfly
+  };
+  static const Rad rads[] = {
+      {2, 1, 0, 0, 0, 4},     {3, 0, 1, 0, 0, 12},    {4, 2, 0, 0, 0, 16},   {5, 0, 0, 1, 0, 34},
+      {6, 1, 1, 0, 0, 36},    {7, 0, 0, 0, 1, 70},    {8, 3, 0, 0, 0, 52},   {9, 0, 2, 0, 0, 96},
+      {10, 1, 0, 1, 0, 88},   {12, 2, 1, 0, 0, 96},   {14, 1, 0, 0, 1, 168}, {15, 0, 1, 1, 0, 162},
+      {16, 4, 0, 0, 0, 144},  {18, 1, 2, 0, 0, 228},  {20, 2, 0, 1, 0, 216}, {21, 0, 1, 0, 1, 294},
+      {24, 3, 1, 0, 0, 252},  {25, 0, 0, 2, 0, 436},  {28, 2, 0, 0, 1, 392}};
+  maxrad = std::min(maxrad, env_int("SHAFFT_AMD_MAXRAD", maxrad));  // can only lower it
+  auto cost = [](const Rad& d) {
+    const int R = d.r;
+    const double per = (d.ops + 4.0 * (R - 1) + 8.0 * R + 10.0) / R;
+    const double rounds = R > 16 ? 1.0 : (double)((16 + R - 1) / R) * R / 16.0;
+    return per * rounds;
+  };
+  // DP over exponent vectors (small: n <= 2^31).
+  const int A = e[0] + 1, B = e[1] + 1, C5 = e[2] + 1, D = e[3] + 1;
+  auto id = [&](int a, int b, int c, int d) { return ((a * B + b) * C5 + c) * D + d; };
+  const int S = A * B * C5 * D;
+  std::vector<double> best(S, 1e300);
+  std::vector<int> choice(S, -1);
+  best[0] = 0.0;
+  for (int a = 0; a < A; ++a)
+    for (int b = 0; b < B; ++b)
+      for (int c = 0; c < C5; ++c)
+        for (int d = 0; d < D; ++d) {
+          const int k = id(a, b, c, d);
+          if (k == 0) continue;
+          for (int j = 0; j < (int)(sizeof(rads) / sizeof(rads[0])); ++j) {
+            const Rad& q = rads[j];
+            if (q.r > maxrad || q.e2 > a || q.e3 > b || q.e5 > c || q.e7 > d) continue;
+            const int prev = id(a - q.e2, b - q.e3, c - q.e5, d - q.e7);
+            const double v = best[prev] + cost(q);
+            if (v < best[k] - 1e-9) {
+              best[k] = v;
+              choice[k] = j;
+            }
+          }
+        }
+  std::vector<int> r;
+  for (int a = e[0], b = e[1], c = e[2], d = e[3]; id(a, b, c, d) != 0;) {
+    const Rad& q = rads[choice[id(a, b, c, d)]];
+    r.push_back(q.r);
+    a -= q.e2;
+    b -= q.e3;
+    c -= q.e5;
+    d -= q.e7;
+  }
+  std::stable_sort(r.begin(), r.end(), [](int x, int y) {
+    const bool px = (x & (x - 1)) == 0, py = (y & (y - 1)) == 0;
+    if (px != py) return px;
+    return x > y;
+  });
   return r;
 }
 
@@ -461,7 +538,8 @@ void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, 
   const bool fixed_candidate = env_int("SHAFFT_AMD_FIXED", 1) && sa_fixed_supported((int)N) &&
                                blu_h == nullptr && !ld.active() && !st.active();
   std::vector<int> rad =
-      N > 1 ? (fixed_candidate ? fixed_radix_plan(N) : radix_plan(N)) : std::vector<int>{};
+      N > 1 ? (fixed_candidate ? fixed_radix_plan(N) : radix_plan(N, prec_ ? 28 : 21))
+            : std::vector<int>{};
   if (rad.size() > SA_MAX_STAGES) fail(ERR_INTERNAL, "too many stages");
   std::vector<int> offs;
   p.tw = stage_table(rad, offs);
@@ -472,6 +550,7 @@ void PassProgram::emit_single(int64_t N, std::vector<Digit> lines, int64_t iks, 
     p.ns[s] = (int32_t)Ns;
     p.twoff[s] = offs[s];
     p.nsdiv[s] = make_fastdiv((uint32_t)Ns);
+    p.nbdiv[s] = make_fastdiv((uint32_t)(N / rad[s]));
     Ns *= rad[s];
   }
   p.ptw_level = pt_level;

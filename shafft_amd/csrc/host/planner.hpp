@@ -90,8 +90,9 @@ int64_t lds_cap(int prec);
 // Longest line for a pass with strided elements (coalescing bound).
 int64_t lf_cap(int prec);
 bool is_smooth7(int64_t n);
-// radices for a 7-smooth n (product == n)
-std::vector<int> radix_plan(int64_t n);
+// radices for a 7-smooth n (product == n), none above maxrad (the generic
+// kernel has radices up to 21 for complex float, 28 for complex double)
+std::vector<int> radix_plan(int64_t n, int maxrad = 28);
 
 class PassProgram {
  public:

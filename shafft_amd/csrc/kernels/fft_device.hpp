@@ -1,5 +1,5 @@
 // Device-side building blocks for the gfx950 FFT kernels: complex arithmetic,
-// fixed-radix DFT butterflies (2,3,4,5,6,7,8,9,10,12,16), and 32-bit division by a
+// fixed-radix DFT butterflies (2..10,12,14,15,16,18,20,21,24,25,28), and 32-bit division by a
 // run-time invariant.
 //
 // Conventions
@@ -278,6 +278,51 @@ __device__ __forceinline__ void dft9(cpx<R>* v) {
   }
 }
 
+// Radix 25 = 5 x 5 by Cooley-Tukey (n = 5 n1 + n2, k = k1 + 5 k2) with the
+// sixteen twiddles w_25^m, m = n2 k1 in 1..16: cos / sin(2 pi m / 25).
+struct Tw25 {
+  static constexpr double c[17] = {
+      1.0, 0.9685831611286311194902, 0.8763066800438635873081,
+      0.7289686274214115231467, 0.5358267949789966182713, 0.3090169943749474241023,
+      0.06279051952931337607618, -0.1873813145857246305426, -0.4257792915650726488625,
+      -0.6374239897486897101767, -0.8090169943749474241023, -0.9297764858882514036609,
+      -0.9921147013144778310498, -0.9921147013144778310498, -0.9297764858882514036609,
+      -0.8090169943749474241023, -0.6374239897486897101767};
+  static constexpr double s[17] = {
+      0.0, 0.2486898871648547882423, 0.4817536741017152749872,
+      0.6845471059286886737323, 0.8443279255020150785486, 0.9510565162951535721164,
+      0.9980267284282715619523, 0.9822872507286886810856, 0.9048270524660195277137,
+      0.770513242775789230803, 0.5877852522924731291687, 0.3681245526846779591569,
+      0.1253332335643042453731, -0.1253332335643042453731, -0.3681245526846779591569,
+      -0.5877852522924731291687, -0.770513242775789230803};
+};
+
+template <int SIGN, typename R>
+__device__ __forceinline__ void dft25(cpx<R>* v) {
+  cpx<R> y[5][5];
+#pragma unroll
+  for (int n2 = 0; n2 < 5; ++n2) {
+    cpx<R> t[5];
+#pragma unroll
+    for (int n1 = 0; n1 < 5; ++n1) t[n1] = v[5 * n1 + n2];
+    dft<5, SIGN>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 5; ++k1) {
+      const int m = n2 * k1;
+      y[n2][k1] = m == 0 ? t[k1] : crot<SIGN>(t[k1], R(Tw25::c[m]), R(Tw25::s[m]));
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 5; ++k1) {
+    cpx<R> t[5];
+#pragma unroll
+    for (int n2 = 0; n2 < 5; ++n2) t[n2] = y[n2][k1];
+    dft<5, SIGN>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 5; ++k2) v[k1 + 5 * k2] = t[k2];
+  }
+}
+
 template <int RAD, int SIGN, typename R>
 __device__ __forceinline__ void dft(cpx<R>* v) {
   if constexpr (RAD == 2) dft2<SIGN>(v);
@@ -288,6 +333,14 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
   else if constexpr (RAD == 10) dft_pfa<2, 5, SIGN>(v);
   else if constexpr (RAD == 12) dft_pfa<4, 3, SIGN>(v);
   else if constexpr (RAD == 9) dft9<SIGN>(v);
+  else if constexpr (RAD == 14) dft_pfa<2, 7, SIGN>(v);
+  else if constexpr (RAD == 15) dft_pfa<3, 5, SIGN>(v);
+  else if constexpr (RAD == 18) dft_pfa<2, 9, SIGN>(v);
+  else if constexpr (RAD == 20) dft_pfa<4, 5, SIGN>(v);
+  else if constexpr (RAD == 21) dft_pfa<3, 7, SIGN>(v);
+  else if constexpr (RAD == 24) dft_pfa<8, 3, SIGN>(v);
+  else if constexpr (RAD == 25) dft25<SIGN>(v);
+  else if constexpr (RAD == 28) dft_pfa<4, 7, SIGN>(v);
   else dft_odd<RAD, SIGN>(v);
 }
 

@@ -43,6 +43,7 @@ typedef struct {
   int32_t ns[SA_MAX_STAGES];    /* product of the radices before stage s   */
   int32_t twoff[SA_MAX_STAGES]; /* offset (elements) into `tw` for stage s */
   sa_fastdiv_t nsdiv[SA_MAX_STAGES];
+  sa_fastdiv_t nbdiv[SA_MAX_STAGES]; /* n / radix[s]: butterflies per line */
   const void* tw; /* per-stage forward twiddles, same precision as data   */
 
   int32_t C;         /* lines per workgroup                              */

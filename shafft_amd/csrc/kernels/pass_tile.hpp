@@ -98,6 +98,46 @@ __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdi
   __syncthreads();
 }
 
+// Radices above 16: a line has fewer butterflies (n/RAD) than threads
+// (n/16), so the tile's C*n/RAD butterflies are numbered densely over all
+// C*T threads (butterfly g = t: line g / (n/RAD)), one round, rather than
+// leaving a 1 - 16/RAD share of every line's threads idle.
+template <typename R, int RAD, int SIGN>
+__device__ __forceinline__ void stage_lds_wide(cpx<R>* lds, int stride, int C, int nthr, int n,
+                                               int Ns, sa_fastdiv_t nsd, sa_fastdiv_t nbd,
+                                               const cpx<R>* __restrict__ tw, int t) {
+  static_assert(RAD > 16, "one butterfly per thread");
+  const int nb = n / RAD;
+  const FastDiv fd{nsd.d, nsd.m, nsd.s};
+  const FastDiv fb{nbd.d, nbd.m, nbd.s};
+  const bool ok = t < nthr && t < C * nb;
+  const int c = ok ? (int)fdiv((uint32_t)t, fb) : 0;
+  const int jb = t - c * nb;
+  cpx<R>* line = lds + (size_t)c * stride;
+  cpx<R> v[RAD];
+  if (ok) {
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[r] = line[lp(jb + r * nb)];
+    if (Ns > 1) {
+      const int q = (int)fdiv((uint32_t)jb, fd);
+      const int rem = jb - q * Ns;
+      const cpx<R>* w = tw + rem * (RAD - 1);
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) v[r] = cmul_dir<SIGN>(v[r], w[r - 1]);
+    }
+    dft<RAD, SIGN>(v);
+  }
+  __syncthreads();
+  if (ok) {
+    const int q = (int)fdiv((uint32_t)jb, fd);
+    const int rem = jb - q * Ns;
+    const int idx = q * Ns * RAD + rem;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) line[lp(idx + r * Ns)] = v[r];
+  }
+  __syncthreads();
+}
+
 }  // namespace sa
 #include "mfma_stage.hpp"
 namespace sa {
@@ -156,8 +196,9 @@ __device__ __forceinline__ int stage_lane(uint32_t tl) {
 
 // One Stockham stage of radix `rad` (wave-uniform) on this thread's line.
 template <typename R, int SIGN>
-__device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t nsd,
-                                          const cpx<R>* tws, cpx<R>* line, int tl, int T) {
+__device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t nsd, sa_fastdiv_t nbd,
+                                          const cpx<R>* tws, cpx<R>* line, int tl, int T,
+                                          cpx<R>* lds, int stride, int C, int t) {
   switch (rad) {
     case 2: stage_lds<R, 2, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 3: stage_lds<R, 3, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
@@ -169,9 +210,22 @@ __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t n
     case 9: stage_lds<R, 9, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 10: stage_lds<R, 10, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
     case 12: stage_lds<R, 12, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 16:
-      stage_lds<R, 16, SIGN>(line, n, Ns, nsd, tws, tl, T);
+    case 16: stage_lds<R, 16, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 14: stage_lds<R, 14, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 15: stage_lds<R, 15, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+#define SA_WIDE(RR) \
+    case RR: stage_lds_wide<R, RR, SIGN>(lds, stride, C, C * T, n, Ns, nsd, nbd, tws, t); break;
+    SA_WIDE(18) SA_WIDE(20) SA_WIDE(21)
+#undef SA_WIDE
+    // Radices 24, 25, 28 for complex double only: in the complex-float
+    // kernel they would raise the register count past 5 waves per SIMD.
+#define SA_WIDE_D(RR)                                                                    \
+    case RR:                                                                             \
+      if constexpr (sizeof(R) == 8)                                                      \
+        stage_lds_wide<R, RR, SIGN>(lds, stride, C, C * T, n, Ns, nsd, nbd, tws, t);     \
       break;
+    SA_WIDE_D(24) SA_WIDE_D(25) SA_WIDE_D(28)
+#undef SA_WIDE_D
     default: break;
   }
 }
@@ -195,7 +249,8 @@ __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>
       return;
     }
   }
-  stage_run<R, SIGN>(uni(a.radix[s]), a.n, a.ns[s], a.nsdiv[s], tws, line, tl, a.T);
+  stage_run<R, SIGN>(uni(a.radix[s]), a.n, a.ns[s], a.nsdiv[s], a.nbdiv[s], tws, line, tl, a.T,
+                     lds, a.ldsstride, a.C, stage_lane(threadIdx.x));
 }
 
 template <typename R, int SIGN>

@@ -9,7 +9,9 @@
 #include <cmath>
 #include <complex>
 #include <cstdlib>
+#include <algorithm>
 #include <array>
+#include <iterator>
 #include <cstdio>
 #include <functional>
 #include <map>
@@ -110,6 +112,10 @@ void check_shape(const sa_pass_t& p) {
   if ((int64_t)p.n > (int64_t)emax * p.T) die("n > EMAX * T");
   for (int s = 0; s < p.nstages; ++s) {
     const int rad = p.radix[s];
+    // butterflies the generic kernel has (fft_device.hpp, pass_tile.hpp)
+    static const int kRad[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16, 18, 20, 21, 24, 25, 28};
+    if (std::find(std::begin(kRad), std::end(kRad), rad) == std::end(kRad)) die("radix without a kernel case");
+    if (sizeof(R) == 4 && rad > 21) die("radix above 21 in a complex-float pass");
     const int maxb = (emax + rad - 1) / rad;
     if (!g_fixed_launch && (p.n / rad + p.T - 1) / p.T > maxb)
       die("butterflies per thread exceed register budget");

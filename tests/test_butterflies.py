@@ -1,5 +1,5 @@
 """The kernels' in-register butterflies (shafft_amd/csrc/kernels/fft_device.hpp:
-radices 2..16 incl. the prime-factor 6/10/12 and Cooley-Tukey 9 forms) built
+radices 2..28 incl. the prime-factor and Cooley-Tukey composite forms) built
 for the host with g++ and checked against the direct DFT, both directions,
 float and double.  The header's one device intrinsic, __umulhi, gets its host
 definition (the high 32 bits of the 64-bit product)."""
@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-RADICES = (2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 16)
+RADICES = (2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16, 18, 20, 21, 24, 25, 28)
 
 SRC = r"""
 #include <cstdint>
@@ -33,7 +33,7 @@ void all(const double* in, double* out) {  // out: [fwd f, bwd f, fwd d, bwd d]
   run<RAD, +1, double>(in, out + 6 * RAD);
 }
 int main() {
-  double in[32], out[128];
+  double in[64], out[256];
   int rad;
   while (std::scanf("%d", &rad) == 1) {
     for (int i = 0; i < 2 * rad; ++i) std::scanf("%lf", &in[i]);
@@ -43,7 +43,11 @@ int main() {
       case 6: all<6>(in, out); break;   case 7: all<7>(in, out); break;
       case 8: all<8>(in, out); break;   case 9: all<9>(in, out); break;
       case 10: all<10>(in, out); break; case 12: all<12>(in, out); break;
-      case 16: all<16>(in, out); break; default: return 2;
+      case 16: all<16>(in, out); break; case 14: all<14>(in, out); break;
+      case 15: all<15>(in, out); break; case 18: all<18>(in, out); break;
+      case 20: all<20>(in, out); break; case 21: all<21>(in, out); break;
+      case 24: all<24>(in, out); break; case 25: all<25>(in, out); break;
+      case 28: all<28>(in, out); break; default: return 2;
     }
     for (int i = 0; i < 8 * rad; ++i) std::printf("%.17g\n", out[i]);
   }
