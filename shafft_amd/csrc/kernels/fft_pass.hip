@@ -21,9 +21,9 @@
 namespace sa {
 
 // Register budget and occupancy.  With the stage loop's LDS addresses
-// computed inside each stage (stage_lane), the LDS-mode kernels need ~75
-// (complex float) / ~127 (complex double) VGPRs, 6 / 4 waves per SIMD, for
-// every radix up to 16.  The planner's tiles hold at most 64 KiB of data
+// computed inside each stage (stage_lane), the LDS-mode kernels need ~94
+// (complex float, radices up to 21) / ~151 (complex double, up to 28) VGPRs,
+// 5 / 3 waves per SIMD.  The planner's tiles hold at most 64 KiB of data
 // (512 threads complex float, 256 complex double), so several independent
 // tiles share a CU and one's HBM loads overlap another's LDS stages; the
 // stages themselves are VALU-issue bound (~120 VALU instructions per element
