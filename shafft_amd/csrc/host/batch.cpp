@@ -118,6 +118,7 @@ void BatchPlan::build() {
       while (C > 1 && (int64_t)C * stride * esz > budget) --C;
       if (C > p.batch) C = (int)std::max<int64_t>(1, p.batch);
       p.T = T;
+      p.emax = 16;
       p.C = C;
       p.ldsstride = stride;
       p.Tdiv = make_fastdiv((uint32_t)T);

@@ -330,7 +330,15 @@ static bool pow2_le16(int r) { return r == 2 || r == 4 || r == 8 || r == 16; }
 // LDS stride and the register-resident stage flags.
 static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
   const int n = p.n;
-  const int emax = 16;
+  // 8 elements per thread for long complex-float in-LDS Bluestein rows (the
+  // middle pass of the three-pass form: few rows, 2x the threads per row;
+  // 1,000,003-point FFT1D 0.062 -> 0.059 ms).  Complex double keeps 16: its
+  // row kernel needs 177 VGPRs, and 512-thread rows would halve the
+  // workgroups per CU (0.079 -> 0.093 ms).
+  const int emax = bluestein && elem_bytes == 8 && n >= 2048 && n <= 8 * 512 &&
+                           env_int("SHAFFT_AMD_BLUE_E8", 1)
+                       ? 8
+                       : 16;
   int T = (n + emax - 1) / emax;
   if (T < 1) T = 1;
   if (T > 512) fail(ERR_INTERNAL, "pass length exceeds the per-workgroup thread budget");
@@ -362,6 +370,7 @@ static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
   if ((int64_t)C * stride * elem_bytes > budget) fail(ERR_INTERNAL, "pass does not fit LDS");
   if (C > p.batch) C = (int)std::max<int64_t>(1, p.batch);
   p.T = T;
+  p.emax = emax;
   p.C = C;
   p.ldsstride = stride;
   p.Tdiv = make_fastdiv((uint32_t)T);

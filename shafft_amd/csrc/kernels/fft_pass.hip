@@ -28,27 +28,29 @@ namespace sa {
 // tiles share a CU and one's HBM loads overlap another's LDS stages; the
 // stages themselves are VALU-issue bound (~120 VALU instructions per element
 // over 3 stages, load and store, by SQ counters).
-template <int SIGN, int MODE, bool TWL>
+// E: elements per thread (8 only for the in-LDS Bluestein rows of the
+// three-pass long Bluestein: twice the threads on a pass of few, long rows).
+template <int SIGN, int MODE, bool TWL, int E = EMAX>
 __global__ __launch_bounds__(512) void fft_lds_pass_f(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<float, SIGN, MODE, 0, TWL>(a, blockIdx.x, reinterpret_cast<cpx<float>*>(lds_raw));
+  pass_tile<float, SIGN, MODE, 0, TWL, E>(a, blockIdx.x, reinterpret_cast<cpx<float>*>(lds_raw));
 }
 
-template <int SIGN, int MODE, bool TWL>
+template <int SIGN, int MODE, bool TWL, int E = EMAX>
 __global__ __launch_bounds__(512) void fft_lds_pass_d(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<double, SIGN, MODE, 0, TWL>(a, blockIdx.x, reinterpret_cast<cpx<double>*>(lds_raw));
+  pass_tile<double, SIGN, MODE, 0, TWL, E>(a, blockIdx.x, reinterpret_cast<cpx<double>*>(lds_raw));
 }
 
-template <typename R, int SIGN, int MODE, bool TWL>
+template <typename R, int SIGN, int MODE, bool TWL, int E = EMAX>
 struct PassKernel;
-template <int SIGN, int MODE, bool TWL>
-struct PassKernel<float, SIGN, MODE, TWL> {
-  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_f<SIGN, MODE, TWL>); }
+template <int SIGN, int MODE, bool TWL, int E>
+struct PassKernel<float, SIGN, MODE, TWL, E> {
+  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_f<SIGN, MODE, TWL, E>); }
 };
-template <int SIGN, int MODE, bool TWL>
-struct PassKernel<double, SIGN, MODE, TWL> {
-  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_d<SIGN, MODE, TWL>); }
+template <int SIGN, int MODE, bool TWL, int E>
+struct PassKernel<double, SIGN, MODE, TWL, E> {
+  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_d<SIGN, MODE, TWL, E>); }
 };
 
 template <typename R>
@@ -63,22 +65,22 @@ __global__ __launch_bounds__(256) void scale_kernel(cpx<R>* __restrict__ x, R f,
 }
 
 // TWL: the stage twiddles are staged in LDS behind the tile (a.tw_lds > 0).
-template <typename R, int SIGN, int MODE, bool TWL>
+template <typename R, int SIGN, int MODE, bool TWL, int E = EMAX>
 static hipError_t launch_t(const sa_pass_t* p, hipStream_t st) {
   const uint32_t nthr = (uint32_t)(p->C * p->T);
   const uint64_t nblk = ((uint64_t)p->batch + p->C - 1) / p->C;
   const size_t lds = ((size_t)p->C * p->ldsstride + (TWL ? (size_t)p->tw_lds : 0)) * sizeof(cpx<R>);
   static const bool attr = [] {  // once per instantiation, thread-safe
-    (void)hipFuncSetAttribute(PassKernel<R, SIGN, MODE, TWL>::fn(),
+    (void)hipFuncSetAttribute(PassKernel<R, SIGN, MODE, TWL, E>::fn(),
                               hipFuncAttributeMaxDynamicSharedMemorySize, sa_max_lds_bytes());
     (void)hipGetLastError();
     return true;
   }();
   (void)attr;
   if constexpr (sizeof(R) == 4)
-    hipLaunchKernelGGL((fft_lds_pass_f<SIGN, MODE, TWL>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
+    hipLaunchKernelGGL((fft_lds_pass_f<SIGN, MODE, TWL, E>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
   else
-    hipLaunchKernelGGL((fft_lds_pass_d<SIGN, MODE, TWL>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
+    hipLaunchKernelGGL((fft_lds_pass_d<SIGN, MODE, TWL, E>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
   return hipGetLastError();
 }
 
@@ -89,7 +91,11 @@ static hipError_t launch_tw(const sa_pass_t* p, hipStream_t st) {
 
 template <typename R>
 static hipError_t launch_prec(const sa_pass_t* p, int sign, hipStream_t st) {
-  if (p->blu_h) return launch_t<R, -1, MODE_BLUE, false>(p, st);
+  if (p->blu_h) {
+    if constexpr (sizeof(R) == 4)  // 8 elements per thread: complex float only
+      if (p->emax == 8) return launch_t<R, -1, MODE_BLUE, false, 8>(p, st);
+    return launch_t<R, -1, MODE_BLUE, false>(p, st);
+  }
   const bool regs = p->reg_first && p->reg_last;
   if (sign < 0) return regs ? launch_tw<R, -1, MODE_REGS>(p, st) : launch_tw<R, -1, MODE_LDS>(p, st);
   return regs ? launch_tw<R, +1, MODE_REGS>(p, st) : launch_tw<R, +1, MODE_LDS>(p, st);
@@ -102,7 +108,11 @@ extern "C" int sa_max_lds_bytes(void) { return 160 * 1024; }
 namespace sa {
 template <typename R>
 static const void* pass_kernel(const sa_pass_t* p, bool twl) {
-  if (p->blu_h) return PassKernel<R, -1, MODE_BLUE, false>::fn();
+  if (p->blu_h) {
+    if constexpr (sizeof(R) == 4)
+      if (p->emax == 8) return PassKernel<R, -1, MODE_BLUE, false, 8>::fn();
+    return PassKernel<R, -1, MODE_BLUE, false>::fn();
+  }
   if (p->reg_first && p->reg_last)
     return twl ? PassKernel<R, -1, MODE_REGS, true>::fn() : PassKernel<R, -1, MODE_REGS, false>::fn();
   return twl ? PassKernel<R, -1, MODE_LDS, true>::fn() : PassKernel<R, -1, MODE_LDS, false>::fn();

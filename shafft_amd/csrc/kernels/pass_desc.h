@@ -48,6 +48,7 @@ typedef struct {
 
   int32_t C;         /* lines per workgroup                              */
   int32_t T;         /* threads per line                                 */
+  int32_t emax;      /* generic kernel: elements per thread (16, or 8 for long Bluestein rows) */
   int32_t ld_kfast;  /* 1: lanes walk k on load, 0: lanes walk lines     */
   int32_t st_kfast;  /* same for the store                               */
   int32_t ldsstride; /* complex elements per line slot in LDS (padded)   */

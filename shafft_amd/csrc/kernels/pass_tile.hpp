@@ -60,10 +60,10 @@ __device__ __forceinline__ void gstore(cpx<R>* p, cpx<R> x, bool nt) {
 // ---------------------------------------------------------------------------
 
 // LDS -> LDS, butterflies jb = tl + i*T held in registers across a barrier.
-template <typename R, int RAD, int SIGN>
+template <typename R, int RAD, int SIGN, int E = EMAX>
 __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdiv_t nsd,
                                           const cpx<R>* __restrict__ tw, int tl, int T) {
-  constexpr int MAXB = (EMAX + RAD - 1) / RAD;
+  constexpr int MAXB = (E + RAD - 1) / RAD;
   const int nb = n / RAD;
   const FastDiv fd{nsd.d, nsd.m, nsd.s};
   cpx<R> v[MAXB][RAD];
@@ -195,24 +195,24 @@ __device__ __forceinline__ int stage_lane(uint32_t tl) {
 }
 
 // One Stockham stage of radix `rad` (wave-uniform) on this thread's line.
-template <typename R, int SIGN>
+template <typename R, int SIGN, int E = EMAX>
 __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t nsd, sa_fastdiv_t nbd,
                                           const cpx<R>* tws, cpx<R>* line, int tl, int T,
                                           cpx<R>* lds, int stride, int C, int t) {
   switch (rad) {
-    case 2: stage_lds<R, 2, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 3: stage_lds<R, 3, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 4: stage_lds<R, 4, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 5: stage_lds<R, 5, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 6: stage_lds<R, 6, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 7: stage_lds<R, 7, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 8: stage_lds<R, 8, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 9: stage_lds<R, 9, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 10: stage_lds<R, 10, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 12: stage_lds<R, 12, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 16: stage_lds<R, 16, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 14: stage_lds<R, 14, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
-    case 15: stage_lds<R, 15, SIGN>(line, n, Ns, nsd, tws, tl, T); break;
+    case 2: stage_lds<R, 2, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 3: stage_lds<R, 3, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 4: stage_lds<R, 4, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 5: stage_lds<R, 5, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 6: stage_lds<R, 6, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 7: stage_lds<R, 7, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 8: stage_lds<R, 8, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 9: stage_lds<R, 9, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 10: stage_lds<R, 10, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 12: stage_lds<R, 12, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 16: stage_lds<R, 16, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 14: stage_lds<R, 14, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 15: stage_lds<R, 15, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
 #define SA_WIDE(RR) \
     case RR: stage_lds_wide<R, RR, SIGN>(lds, stride, C, C * T, n, Ns, nsd, nbd, tws, t); break;
     SA_WIDE(18) SA_WIDE(20) SA_WIDE(21)
@@ -233,7 +233,7 @@ __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t n
 // MX != 0 (complex float only): radix-8/16 stages on the matrix cores in
 // split precision (mfma_stage.hpp); `lds` is the tile's LDS base.
 // twb: the stage twiddle table (global, or staged in LDS by the caller).
-template <typename R, int SIGN, int MX = 0>
+template <typename R, int SIGN, int MX = 0, int E = EMAX>
 __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>* line, int tl,
                                                cpx<R>* lds, const cpx<R>* twb) {
   const cpx<R>* tws = twb + a.twoff[s];
@@ -249,7 +249,7 @@ __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>
       return;
     }
   }
-  stage_run<R, SIGN>(uni(a.radix[s]), a.n, a.ns[s], a.nsdiv[s], a.nbdiv[s], tws, line, tl, a.T,
+  stage_run<R, SIGN, E>(uni(a.radix[s]), a.n, a.ns[s], a.nsdiv[s], a.nbdiv[s], tws, line, tl, a.T,
                      lds, a.ldsstride, a.C, stage_lane(threadIdx.x));
 }
 
@@ -322,11 +322,11 @@ __device__ __forceinline__ uint32_t slot_elem(const sa_pass_t& a, uint32_t t, bo
   return fdiv_d(t, a.Cdiv);
 }
 
-// Loads elements k = k0 + i*T (i < EMAX, k < n) of one line into v, with the
+// Loads elements k = k0 + i*T (i < E, k < n) of one line into v, with the
 // load hooks applied (zero outside the hook's limit).
-template <typename R>
+template <typename R, int E = EMAX>
 __device__ __forceinline__ void load_line(const sa_pass_t& a, const LineOff& o, uint32_t k0,
-                                          cpx<R> (&v)[EMAX]) {
+                                          cpx<R> (&v)[E]) {
   const int n = a.n;
   const uint32_t T = (uint32_t)a.T;
   const cpx<R>* __restrict__ src = reinterpret_cast<const cpx<R>*>(a.in) + o.in;
@@ -334,7 +334,7 @@ __device__ __forceinline__ void load_line(const sa_pass_t& a, const LineOff& o, 
   const bool plain = a.pre == nullptr && a.ld_level < 0 && a.ld_kmul == 1 && a.ld_limit >= n;
   if (plain) {
 #pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
+    for (int i = 0; i < E; ++i) {
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) v[i] = gload(src + (int64_t)k * a.iks, nt);
     }
@@ -342,7 +342,7 @@ __device__ __forceinline__ void load_line(const sa_pass_t& a, const LineOff& o, 
     const cpx<R>* __restrict__ pre = reinterpret_cast<const cpx<R>*>(a.pre);
     const int64_t base = o.ld_base;
 #pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
+    for (int i = 0; i < E; ++i) {
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
         const int64_t m = (int64_t)k * a.ld_kmul + base;
@@ -358,9 +358,9 @@ __device__ __forceinline__ void load_line(const sa_pass_t& a, const LineOff& o, 
 }
 
 // Stores elements k = k0 + i*T of one line from v with the store hooks.
-template <typename R, int SIGN>
+template <typename R, int SIGN, int E = EMAX>
 __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o, uint32_t k0,
-                                           const cpx<R> (&v)[EMAX]) {
+                                           const cpx<R> (&v)[E]) {
   const int n = a.n;
   const uint32_t T = (uint32_t)a.T;
   cpx<R>* __restrict__ dst = reinterpret_cast<cpx<R>*>(a.out) + o.out;
@@ -384,17 +384,17 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
     const cpx<double> step = w64((dig * T) % N);
     const R cs = a.ptw_conj ? R(-1) : R(1);  // conj(W) on request
 #pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
+    for (int i = 0; i < E; ++i) {
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
         cpx<R> x = cmul_dir<SIGN>(v[i], cpx<R>{(R)w.x, cs * (R)w.y});
         gstore(dst + (int64_t)k * a.oks, x, nt);
       }
-      if (i + 1 < EMAX) w = cmul(w, step);
+      if (i + 1 < E) w = cmul(w, step);
     }
   } else if (post == nullptr && a.st_level < 0 && a.st_kmul == 1 && a.st_limit >= n) {
 #pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
+    for (int i = 0; i < E; ++i) {
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
         cpx<R> x = v[i];
@@ -404,7 +404,7 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
   } else {
     const int64_t base = o.st_base;
 #pragma unroll
-    for (int i = 0; i < EMAX; ++i) {
+    for (int i = 0; i < E; ++i) {
       const uint32_t k = k0 + i * T;
       if (k < (uint32_t)n) {
         const int64_t m = (int64_t)k * a.st_kmul + base;
@@ -436,8 +436,9 @@ __device__ __forceinline__ const cpx<R>* stage_twiddles_to_lds(const sa_pass_t& 
 // One tile (C consecutive lines) of a pass, run by the whole workgroup.
 // Threads t >= C*T of a larger workgroup idle through the barriers.
 // TWL: the stages read their twiddles from LDS (a.tw_lds > 0, not Bluestein).
-template <typename R, int SIGN, int MODE, int MX = 0, bool TWL = false>
+template <typename R, int SIGN, int MODE, int MX = 0, bool TWL = false, int E = EMAX>
 __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx<R>* lds) {
+  static_assert(E == EMAX || MODE != MODE_REGS, "register stages hold EMAX elements");
   constexpr bool BLUE = MODE == MODE_BLUE;
 
   const uint32_t nthr = (uint32_t)(a.C * a.T);
@@ -455,7 +456,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   // loops skip them; they still take part in every barrier.
   const uint32_t tl = active ? slot_elem(a, t, id_kfast) : (uint32_t)n;
   cpx<R>* line = lds + (size_t)c * a.ldsstride;
-  cpx<R> v[EMAX];
+  cpx<R> v[E];
   static_assert(!(TWL && BLUE), "Bluestein passes read their twiddles from global memory");
   const cpx<R>* twb = reinterpret_cast<const cpx<R>*>(a.tw);
 
@@ -467,7 +468,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
       load_line<R>(a, o, tl, v);
     } else {
 #pragma unroll
-      for (int i = 0; i < EMAX; ++i) v[i] = {R(0), R(0)};
+      for (int i = 0; i < E; ++i) v[i] = {R(0), R(0)};
     }
     if constexpr (TWL) twb = stage_twiddles_to_lds<R>(a, lds);
     first_dispatch<R, SIGN>(a, v, line, (int)tl, active);
@@ -476,16 +477,16 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
     const uint32_t k0 = slot_elem(a, t, a.ld_kfast != 0);
     const uint32_t b = line0 + lc;
     const bool have = active && b < (uint32_t)a.batch;
-    cpx<R> u[EMAX];
+    cpx<R> u[E];
     if (have) {
       const LineOff o = line_offsets(a, b);
-      load_line<R>(a, o, k0, u);
+      load_line<R, E>(a, o, k0, u);
     }
     if constexpr (TWL) twb = stage_twiddles_to_lds<R>(a, lds);
     if (have) {
       cpx<R>* ll = lds + (size_t)lc * a.ldsstride;
 #pragma unroll
-      for (int i = 0; i < EMAX; ++i) {
+      for (int i = 0; i < E; ++i) {
         const uint32_t k = k0 + i * (uint32_t)a.T;
         if (k < (uint32_t)n) ll[lp((int)k)] = u[i];
       }
@@ -497,11 +498,11 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   if constexpr (!BLUE) {
     const int s0 = reg_first ? 1 : 0;
     const int s1 = reg_last ? uni(a.nstages) - 1 : uni(a.nstages);
-    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX>(a, s, line, stage_lane(tl), lds, twb);
+    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX, E>(a, s, line, stage_lane(tl), lds, twb);
     if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl, active, twb);
   } else {
     const int nst = uni(a.nstages);
-    for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0>(a, s, line, stage_lane(tl), lds, twb);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0, E>(a, s, line, stage_lane(tl), lds, twb);
     const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
     if (a.blu_level >= 0 && active) {
       const uint32_t b = line0 + c;
@@ -515,7 +516,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
     }
     for (int k = (int)tl; k < n; k += a.T) line[lp(k)] = cmul(line[lp(k)], h[k]);
     __syncthreads();
-    for (int s = 0; s < nst; ++s) stage_dispatch<R, +1, 0>(a, s, line, stage_lane(tl), lds, twb);
+    for (int s = 0; s < nst; ++s) stage_dispatch<R, +1, 0, E>(a, s, line, stage_lane(tl), lds, twb);
   }
 
   // ---- store -----------------------------------------------------------------
@@ -531,14 +532,14 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
     const uint32_t b = line0 + sc;
     if (active && b < (uint32_t)a.batch) {
       const cpx<R>* ll = lds + (size_t)sc * a.ldsstride;
-      cpx<R> u[EMAX];
+      cpx<R> u[E];
 #pragma unroll
-      for (int i = 0; i < EMAX; ++i) {
+      for (int i = 0; i < E; ++i) {
         const uint32_t k = k0 + i * (uint32_t)a.T;
         if (k < (uint32_t)n) u[i] = ll[lp((int)k)];
       }
       const LineOff o = line_offsets(a, b);
-      store_line<R, SIGN>(a, o, k0, u);
+      store_line<R, SIGN, E>(a, o, k0, u);
     }
   }
 }

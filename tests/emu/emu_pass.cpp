@@ -110,6 +110,12 @@ void check_shape(const sa_pass_t& p) {
   const int emax = g_fixed_launch && sizeof(R) == 4 && (p.n == 512 || p.n == 1024) ? 32 : 16;
   if (p.T < 1 || p.C < 1 || p.C * p.T > 512) die("threads per workgroup");
   if ((int64_t)p.n > (int64_t)emax * p.T) die("n > EMAX * T");
+  if (!g_fixed_launch) {
+    if (p.emax != 8 && p.emax != 16) die("elements per thread");
+    if (p.emax == 8 && (!p.blu_h || sizeof(R) != 4))
+      die("8 elements per thread outside the complex-float Bluestein row kernel");
+    if ((int64_t)p.n > (int64_t)p.emax * p.T) die("n > emax * T");
+  }
   for (int s = 0; s < p.nstages; ++s) {
     const int rad = p.radix[s];
     // butterflies the generic kernel has (fft_device.hpp, pass_tile.hpp)
