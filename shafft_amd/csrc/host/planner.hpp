@@ -7,10 +7,12 @@
 //     the N1-point DFTs with the W_N^{n2 k1} twiddle fused into its store,
 //     then the N2-point sub-problem runs with the (k1, k2) -> k1 + N1 k2
 //     transpose fused into its final store.  Recursive.
-//   * N too long, not smooth             -> Bluestein through a two-level
-//     length-M (power of two) four-step: chirp + zero padding fused into the
-//     first load, the filter product into the second store, the output chirp
-//     and truncation into the last store (4 passes over M).
+//   * N too long, not smooth             -> Bluestein over a length M = N1*N2
+//     (power of two) in three passes: strided N1-point columns with the chirp
+//     and zero padding fused into the load; the N2-point rows doing forward
+//     DFT, filter product and inverse DFT in LDS (spectrum left transposed);
+//     inverse N1-point columns with the output chirp and truncation fused into
+//     the store.  Falls back to two separate M-point four-step chains.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
