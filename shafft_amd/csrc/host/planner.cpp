@@ -776,12 +776,11 @@ void PassProgram::add_transform_impl(int64_t N, const std::vector<Digit>& lines,
   const int64_t M = next_pow2(2 * N - 1);
   // Direction-dependent chirp tables: [0] forward, [1] backward.
   std::vector<cld> chirp[2], H[2];
-  bluestein_tables(N, M, -1, chirp[0], H[0]);
-  bluestein_tables(N, M, +1, chirp[1], H[1]);
+  bluestein_tables_pair(N, M, chirp, H);
   const void* tc[2] = {upload(chirp[0]), upload(chirp[1])};
-  const void* th[2] = {upload(H[0]), upload(H[1])};
   const size_t first = passes_.size();
   if (M <= cap) {
+    const void* th[2] = {upload(H[0]), upload(H[1])};
     ElemHook ld, st;
     ld.table = tc[0];
     ld.limit = N;
@@ -804,6 +803,9 @@ void PassProgram::add_transform_impl(int64_t N, const std::vector<Digit>& lines,
       bluestein_three_pass(N, M, lines, tc, H)) {
     return;
   }
+  // The filter in natural order (the three-pass form uploads its own
+  // transposed copy).
+  const void* th[2] = {upload(H[0]), upload(H[1])};
   // Long Bluestein: forward M-FFT (chirp + zero padding on its first load,
   // filter product on its last store), then backward M-FFT (output chirp and
   // truncation on its last store).  The intermediate is [lines][M][u] as

@@ -34,7 +34,14 @@ void two_level(int64_t N, int& shift, std::vector<cld>& lo, std::vector<cld>& hi
 void bluestein_tables(int64_t n, int64_t M, int sign, std::vector<cld>& chirp,
                       std::vector<cld>& H);
 
-// In-place radix-2 FFT in long double (forward sign), M a power of two.
+// Both directions at once ([0] forward, [1] backward).  h is even
+// (h[m] = h[M-m]), so its transform is even and the backward tables are the
+// conjugates of the forward ones: chirp_b = conj(chirp_f) and
+// H_b[k] = conj(H_f[-k]) = conj(H_f[k]).  One transform instead of two.
+void bluestein_tables_pair(int64_t n, int64_t M, std::vector<cld> chirp[2], std::vector<cld> H[2]);
+
+// In-place radix-2 FFT in long double (forward sign), M a power of two;
+// large transforms run on up to 16 host threads.
 void fft_ld(std::vector<cld>& a);
 
 }  // namespace sa

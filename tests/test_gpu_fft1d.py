@@ -3,8 +3,10 @@
 Sizes follow tests/integration/cpp/test_FFT1D_roundtrip.cpp (delta at 256,
 round trips at 128, sizes {32,33,64,100,128}, non-divisible {10,17,31,100,
 127}); the headline lengths 2^27 (four-step, 3 LDS passes) and the prime
-1,000,003 (Bluestein through a 2^21 convolution, 4 passes) are checked by
-direct-summation spot bins, Parseval and round trips.
+1,000,003 (Bluestein through a 2^21 convolution, 3 passes), and the prime
+4,194,319 (a 2^24 convolution, past the three-pass form: two M-point
+four-step chains) are checked by direct-summation spot bins, Parseval and
+round trips.
 """
 import numpy as np
 import pytest
@@ -77,7 +79,8 @@ def _energy(a, chunk=1 << 24):
                for s in range(0, a.size, chunk))
 
 
-@pytest.mark.parametrize("n,prec", [(1 << 27, api.C2C), (1 << 27, api.Z2Z), (1000003, api.C2C), (1000003, api.Z2Z)])
+@pytest.mark.parametrize("n,prec", [(1 << 27, api.C2C), (1 << 27, api.Z2Z), (1000003, api.C2C), (1000003, api.Z2Z),
+                                    (4194319, api.C2C), (4194319, api.Z2Z)])
 def test_fft1d_headline_lengths(gpu_lib, n, prec):
     rng = np.random.default_rng(44)
     x = np.empty(n, DT[prec])
