@@ -3,6 +3,7 @@
 #include <cmath>
 #include <algorithm>
 #include <stdexcept>
+#include <system_error>
 #include <thread>
 #include <utility>
 
@@ -91,8 +92,12 @@ void parallel_for(int64_t n, int64_t grain, const F& f) {
   }
   std::vector<std::thread> th;
   th.reserve(nt);
-  for (int64_t t = 0; t < nt; ++t)
-    th.emplace_back([&f, n, nt, t] { f(n * t / nt, n * (t + 1) / nt); });
+  int64_t t = 0;
+  try {
+    for (; t < nt; ++t) th.emplace_back([&f, n, nt, t] { f(n * t / nt, n * (t + 1) / nt); });
+  } catch (const std::system_error&) {
+    f(n * t / nt, n);  // no more threads: the rest on this one
+  }
   for (auto& x : th) x.join();
 }
 
