@@ -502,8 +502,8 @@ def main():
                          "reference kernel for HBM counter scaling under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU per process; multi-GPU runs "
-                                   "are not part of it (see DESIGN.md, sharding)"}))
+        print(json.dumps({"error": "shafft_amd runs one process on one GPU; multi-GPU "
+                                   "(slab-sharded) runs are not implemented"}))
         sys.exit(2)
 
     if args.workload == "batch":

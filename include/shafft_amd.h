@@ -5,8 +5,8 @@
  *  1. The backend seam.  SHAFFT's core calls its FFT backend through a small
  *     link-time symbol set (fftndPlan/fftndExecute/..., fft1d*, normalize*).
  *     Each shafftAmd* function below replaces one of them; the C++ adapter
- *     that binds them to the reference's FFTNDHandle / FFT1DHandle types is
- *     shown in INTEGRATION.md.  Plans are opaque handles owned by the
+ *     that binds them to the reference's FFTNDHandle / FFT1DHandle types
+ *     forwards each seam call here.  Plans are opaque handles owned by the
  *     library; data/work buffers are owned by the caller.
  *
  *  2. The reference's public C API (include/shafft/shafft.h) at one rank per

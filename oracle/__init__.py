@@ -1,6 +1,6 @@
 """FP64 oracle for the slab C2C FFT path -- TEST INFRASTRUCTURE ONLY.
 
-Used by tests/, by ``__graft_entry__.smoke()`` as the checker, and by the
+Used by tests/, by ``__graft_entry__.smoke()`` to check its result, and by the
 benchmark's CPU-baseline leg as the timed CPU port.  Nothing in the product
 (``shafft_amd``) imports it.  See dft_oracle.c for what each function
 restates and which reference lines it follows.

@@ -6,7 +6,7 @@
  * tests/unit/test_fft_highdim.cpp:53-58), backward sign +1, and the
  * normalisation factor (1/sqrt(prod dims))^e (src/cpp/shafft_internal.cpp:
  * 129-188).  Only tests, smoke() and the benchmark's CPU-baseline leg use it,
- * and only as the checker or the timed CPU baseline -- never as a product
+ * and only to check results or as the timed CPU baseline -- never as a product
  * path.
  *
  * oracle_dft_axis: direct O(n^2)-per-line DFT ("dft_partial" in

@@ -1,4 +1,4 @@
-"""Pins the FP64 oracle before it is trusted as the checker.
+"""Pins the FP64 oracle before GPU results are checked against it.
 
 Each check is chosen so that a plausible mistake in the oracle (a dropped
 term, a wrong sign, a transposed operand, a wrong index) fails it:
