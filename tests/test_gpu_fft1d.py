@@ -6,7 +6,7 @@ round trips at 128, sizes {32,33,64,100,128}, non-divisible {10,17,31,100,
 1,000,003 (Bluestein through a 2^21 convolution, 3 passes), and the prime
 4,194,319 (a 2^24 convolution, past the three-pass form: two M-point
 four-step chains) are checked by direct-summation spot bins, Parseval and
-round trips.
+round trips; 9 * 2^28 (element indices past 2^31) by an impulse.
 """
 import numpy as np
 import pytest
@@ -99,3 +99,49 @@ def test_fft1d_headline_lengths(gpu_lib, n, prec):
     assert e_bins.max() < TOL[prec] * 10
     assert e_par < TOL[prec]
     assert e_rt < TOL[prec] * np.sqrt(np.log(n))
+
+
+def test_fft1d_past_int32_indices(gpu_lib):
+    """N = 9 * 2^28 = 2,415,919,104 complex64 elements (18 GB per buffer):
+    element indices past 2^31 - 1, beyond the reference's INT_MAX guard on
+    its 1-D lengths (src/cpp/fft/hipfft_method/1d/fft1d_method.hip:416-581).
+    A unit impulse at p = N - 5 must transform to the plane wave
+    W_N^{k p} (spot bins with the phase k*p mod N in exact integers, and
+    Parseval), and forward -> normalize -> backward -> normalize must put it
+    back at p: an index that wrapped anywhere in the passes would move or
+    lose it."""
+    n, prec = 9 << 28, api.C2C
+    p = n - 5
+    x = np.zeros(n, np.complex64)
+    x[p] = 1
+    print("N", n, api.OneDBackendPlan(n, prec).describe(), flush=True)
+    fp = api.FFT1D()
+    fp.init(n, n, 0, prec)
+    fp.plan()
+    a, b = api.DeviceBuffer(n, prec), api.DeviceBuffer(n, prec)
+    a.upload(x)
+    fp.set_buffers(a.ptr, b.ptr)
+    fp.execute(api.FORWARD)
+    d, _ = fp.get_buffers()
+    spec = api.download_ptr(d, n, prec)
+    ks = np.array([0, 1, 12345, (1 << 31) + 17, n // 2 + 7, n - 1], dtype=np.int64)
+    ph = np.array([(int(k) * p) % n for k in ks], dtype=np.float64) / n
+    ref = np.exp(-2j * np.pi * ph)
+    e_bins = np.abs(spec[ks] - ref).max()
+    chunk = 1 << 26
+    e_par = abs(_energy(spec) / n - 1)  # FP64 accumulation
+    del spec
+    fp.normalize()
+    fp.execute(api.BACKWARD)
+    fp.normalize()
+    d, _ = fp.get_buffers()
+    back = api.download_ptr(d, n, prec)
+    peak = complex(back[p])
+    back[p] = 0
+    rest = max(float(np.abs(back[s:s + chunk]).max()) for s in range(0, n, chunk))
+    print(f"N={n}: bins {e_bins:.3e} Parseval {e_par:.3e} back[p] {peak:.7f} max elsewhere {rest:.3e}",
+          flush=True)
+    assert e_bins < 1e-4
+    assert e_par < 1e-5
+    assert abs(peak - 1) < 1e-5
+    assert rest < 1e-5
