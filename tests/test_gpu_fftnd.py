@@ -239,3 +239,50 @@ def test_headline_shapes_properties(gpu_lib, prec, dims):
     print(f"{dims} prec {prec}: Parseval rel {e_par:.3e} roundtrip relL2 {e_rt:.3e}")
     assert e_par < TOL[prec]
     assert e_rt < TOL[prec] * np.sqrt(np.log(n))
+
+
+def test_fftnd_past_int32_indices(gpu_lib):
+    """2304 x 1024 x 1024 complex64 = 2,415,919,104 elements (18 GB per
+    buffer): element indices past 2^31 - 1 through the rotated-layout plan.
+    A unit impulse at p = (2303, 1023, 1021) must transform to the plane
+    wave exp(-2 pi i sum_a k_a p_a / N_a) (spot bins with exact integer
+    phases, Parseval), and forward -> normalize -> backward -> normalize must
+    return it to p: an index that wrapped anywhere would move or lose it."""
+    dims, prec = (2304, 1024, 1024), api.C2C
+    n = int(np.prod(dims))
+    p = (2303, 1023, 1021)
+    flat = (p[0] * dims[1] + p[1]) * dims[2] + p[2]
+    x = np.zeros(n, np.complex64)
+    x[flat] = 1
+    print(api.NDBackendPlan(list(dims), [0, 1, 2], prec).describe(), flush=True)
+    plan = api.FFTND()
+    plan.init([1, 1, 1], dims, prec)
+    plan.plan()
+    a, b = api.DeviceBuffer(n, prec), api.DeviceBuffer(n, prec)
+    a.upload(x)
+    del x
+    plan.set_buffers(a.ptr, b.ptr)
+    plan.execute(api.FORWARD)
+    d, _ = plan.get_buffers()
+    spec = api.download_ptr(d, n, prec).reshape(dims)
+    errs = []
+    for kk in [(0, 0, 0), (1, 2, 3), (2303, 1023, 1023), (1500, 7, 900)]:
+        turns = sum(((k * q) % m) / m for k, q, m in zip(kk, p, dims))
+        errs.append(abs(complex(spec[kk]) - np.exp(-2j * np.pi * turns)))
+    e_par = abs(_energy(spec.ravel()) / n - 1)
+    del spec
+    plan.normalize()
+    plan.execute(api.BACKWARD)
+    plan.normalize()
+    d, _ = plan.get_buffers()
+    back = api.download_ptr(d, n, prec)
+    peak = complex(back[flat])
+    back[flat] = 0
+    chunk = 1 << 26
+    rest = max(float(np.abs(back[s:s + chunk]).max()) for s in range(0, n, chunk))
+    print(f"{dims}: bins {max(errs):.3e} Parseval {e_par:.3e} back[p] {peak:.7f} "
+          f"max elsewhere {rest:.3e}", flush=True)
+    assert max(errs) < 1e-4
+    assert e_par < 1e-5
+    assert abs(peak - 1) < 1e-5
+    assert rest < 1e-5
