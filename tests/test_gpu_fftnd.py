@@ -133,6 +133,21 @@ def test_config1_roundtrip_api(gpu_lib, prec):
     assert oracle.check_rel_error(back, x, TOL[prec])
 
 
+@pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
+@pytest.mark.parametrize("dims", [(1, 1, 1), (1, 64, 1), (32, 1, 17), (1, 1, 4099), (2,), (1, 2, 1, 3)])
+def test_degenerate_extents(gpu_lib, prec, dims):
+    """Unit-length axes (identity transforms), a length-2 line and a prime
+    Bluestein axis between unit axes, through the public API: forward against
+    the FP64 oracle and the normalized round trip."""
+    x = oracle.uniform_input(dims, 7, DT[prec])
+    spec, back = api_roundtrip(dims, prec, x)
+    ref = oracle.dftn(x.astype(np.complex128), None, -1)
+    e1, e2 = oracle.rel_l2(spec, ref), oracle.rel_l2(back, x)
+    print(f"{dims} prec {prec}: forward relL2 {e1:.3e} roundtrip relL2 {e2:.3e}")
+    assert oracle.check_rel_error(spec, ref, TOL[prec])
+    assert oracle.check_rel_error(back, x, TOL[prec])
+
+
 def test_delta_and_plane_wave_closed_forms(gpu_lib):
     dims = (64, 64, 32)
     n = int(np.prod(dims))
