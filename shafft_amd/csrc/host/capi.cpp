@@ -10,6 +10,7 @@
 
 #include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -923,23 +924,28 @@ int shafftNormalize(void* planPtr) {
   });
 }
 
-int shafftAllocBufferF(size_t count, void** buf) {
+// Device allocation of `count` elements of `esz` bytes, as
+// src/cpp/detail/buffer_utils.hpp:37-56: count 0 gives a null buffer and
+// success, a failed (or unrepresentable) allocation ERR_ALLOC.
+static int alloc_elems(size_t count, size_t esz, void** buf) {
   return guarded([&] {
     if (!buf) fail(ERR_NULLPTR, "null output");
     *buf = nullptr;
-    hip_check(hipMalloc(buf, count * 8), "hipMalloc");
+    if (count == 0) return 0;
+    if (count > SIZE_MAX / esz) fail(ERR_ALLOC, "allocation size overflows size_t");
+    const hipError_t e = hipMalloc(buf, count * esz);
+    if (e != hipSuccess) {
+      *buf = nullptr;
+      throw Error(ERR_ALLOC, SRC_HIP, static_cast<int>(e),
+                  std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
     return 0;
   });
 }
 
-int shafftAllocBufferD(size_t count, void** buf) {
-  return guarded([&] {
-    if (!buf) fail(ERR_NULLPTR, "null output");
-    *buf = nullptr;
-    hip_check(hipMalloc(buf, count * 16), "hipMalloc");
-    return 0;
-  });
-}
+int shafftAllocBufferF(size_t count, void** buf) { return alloc_elems(count, 8, buf); }
+
+int shafftAllocBufferD(size_t count, void** buf) { return alloc_elems(count, 16, buf); }
 
 int shafftFreeBufferF(void* buf) {
   return guarded([&] {
@@ -953,6 +959,7 @@ int shafftFreeBufferD(void* buf) { return shafftFreeBufferF(buf); }
 static int copy_bytes(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
   return guarded([&] {
     if (!dst || !src) fail(ERR_NULLPTR, "null pointer");
+    if (bytes == 0) return 0;
     hip_check(hipMemcpy(dst, src, bytes, kind), "hipMemcpy");
     return 0;
   });

@@ -121,6 +121,32 @@ static void test_buffers(void) {
   free(gd);
 }
 
+/* Allocation edge cases of src/cpp/detail/buffer_utils.hpp:37-56 that make
+ * no device call: count 0 is a null buffer and success; a byte count that
+ * does not fit size_t is ERR_ALLOC. */
+static void test_alloc_edges_host(void) {
+  void* b = (void*)&g_fail;
+  check("AllocBufferF(0) -> success, NULL", shafftAllocBufferF(0, &b) == 0 && b == NULL);
+  b = (void*)&g_fail;
+  check("AllocBufferD(SIZE_MAX) -> ALLOC, NULL",
+        shafftAllocBufferD(SIZE_MAX, &b) == SHAFFT_ERR_ALLOC && b == NULL &&
+            shafftLastErrorStatus() == SHAFFT_ERR_ALLOC);
+  check("FreeBuffer(NULL) -> success", shafftFreeBufferF(NULL) == 0);
+  float h[2] = {1.0f, 2.0f};
+  check("CopyToBufferF count 0 -> success", shafftCopyToBufferF((void*)h, h, 0) == 0);
+}
+
+/* An allocation the device cannot hold fails with ERR_ALLOC and leaves the
+ * library usable. */
+static void test_alloc_too_large(void) {
+  void* b = (void*)&g_fail;
+  const size_t huge = (size_t)1 << 42; /* 64 TiB of complex128 */
+  check("AllocBufferD(2^42) -> ALLOC, NULL", shafftAllocBufferD(huge, &b) == SHAFFT_ERR_ALLOC && b == NULL);
+  void* ok = NULL;
+  check("allocation after a failed one", shafftAllocBufferF(1024, &ok) == 0 && ok != NULL);
+  shafftFreeBufferF(ok);
+}
+
 /* ------------------------------------------------------- configuration */
 
 static void test_configuration(void) {
@@ -368,11 +394,13 @@ int main(int argc, char** argv) {
   test_info_and_errors();
   test_life_cycle();
   test_configuration();
+  test_alloc_edges_host();
   if (host_only) {
     printf("%d check(s) failed\n", g_fail);
     return g_fail;
   }
   test_buffers();
+  test_alloc_too_large();
   {
     int d1[3] = {32, 32, 32};
     test_nd_roundtrip(3, d1, SHAFFT_C2C, "N-D round trip 32x32x32 complex64");
