@@ -274,6 +274,108 @@ void shafftGetVersion(int* major, int* minor, int* patch);            /* shafft.
 const char* shafftGetVersionString(void);                             /* shafft.h:834 */
 int shafftFinalize(void);                                             /* shafft.h:845 */
 
+/* Configuration objects (shafft.h:203-475).  The structs keep the
+ * reference's field names and order with MPI_Comm replaced by the one-rank
+ * handle; arrays and strings in them are owned by the config (malloc'd by
+ * Init, freed by Release).  At one rank Resolve gives commDims {1,...,1},
+ * nda 0, commSize 1, both layouts = the whole tensor (offset 0, every axis
+ * contiguous, none distributed), isActive 1, activeRank 0, activeSize 1,
+ * nodeId 0, nodeCount 1; the hostname comes from gethostname and deviceName
+ * from the current HIP device (NULL when there is none).  Flags as in
+ * src/cpp/config_info.cpp:494-501: RESOLVED, plus CHANGED_COMM_DIMS /
+ * CHANGED_NDA when the resolved grid or nda differs from the hints. */
+typedef enum {
+  SHAFFT_CONFIG_CHANGED_COMM_DIMS = (1 << 0),
+  SHAFFT_CONFIG_CHANGED_NDA = (1 << 1),
+  SHAFFT_CONFIG_INACTIVE_RANKS = (1 << 2),
+  SHAFFT_CONFIG_RESOLVED = (1 << 3)
+} shafft_config_flags_t;                                              /* shafft.h:208 */
+typedef struct shafft_nd_layout_t {
+  size_t* subsize;
+  size_t* offset;
+  size_t localElements;
+  int hasLocalElements;
+  int* contiguousAxes;  /* length ndim - nda */
+  int* distributedAxes; /* length nda */
+} shafft_nd_layout_t;                                                 /* shafft.h:224 */
+typedef struct shafft_nd_config_t {
+  size_t structSize; /* sizeof(shafft_nd_config_t) once initialised */
+  int apiVersion;
+  int ndim;
+  shafft_t precision;
+  size_t* globalShape;
+  shafft_decomposition_strategy_t strategy;
+  shafft_transform_layout_t outputPolicy;
+  size_t memLimit;
+  int* hintCommDims;
+  int hintNda;
+  int* commDims;
+  int nda;
+  int commSize;
+  size_t allocElements;
+  shafft_nd_layout_t initial;
+  shafft_nd_layout_t output;
+  int isActive;
+  int activeRank;
+  int activeSize;
+  shafft_comm_t worldComm;
+  shafft_comm_t activeComm;
+  int nodeId;
+  int nodeCount;
+  char* hostname;
+  size_t hostnameLen;
+  char* deviceName;
+  size_t deviceNameLen;
+  int flags;
+  int status;
+  int reserved[8];
+} shafft_nd_config_t;                                                 /* shafft.h:244 */
+typedef struct shafft_1d_layout_t {
+  size_t localSize;
+  size_t localStart;
+  int hasLocalElements;
+} shafft_1d_layout_t;                                                 /* shafft.h:298 */
+typedef struct shafft_1d_config_t {
+  size_t structSize;
+  int apiVersion;
+  size_t globalSize;
+  shafft_t precision;
+  shafft_1d_layout_t initial;
+  shafft_1d_layout_t output;
+  size_t allocElements;
+  int isActive;
+  int activeRank;
+  int activeSize;
+  shafft_comm_t worldComm;
+  shafft_comm_t activeComm;
+  int nodeId;
+  int nodeCount;
+  char* hostname;
+  size_t hostnameLen;
+  char* deviceName;
+  size_t deviceNameLen;
+  int flags;
+  int status;
+  int reserved[8];
+} shafft_1d_config_t;                                                 /* shafft.h:313 */
+/* cfg must be zero-initialised; a second Init without Release is
+ * ERR_INVALID_STATE; cComm must be 0 (ERR_INVALID_COMM). */
+int shafftConfigNDInit(shafft_nd_config_t* cfg, int ndim, const size_t* globalShape,
+                       shafft_t precision, const int* commDims, int hintNda,
+                       shafft_decomposition_strategy_t strategy,
+                       shafft_transform_layout_t outputPolicy, size_t memLimit,
+                       shafft_comm_t cComm);                          /* shafft.h:372 */
+void shafftConfigNDRelease(shafft_nd_config_t* cfg);                  /* shafft.h:394 */
+int shafftConfigNDResolve(shafft_nd_config_t* cfg);                   /* shafft.h:406 */
+int shafftNDInitFromConfig(void* planPtr, shafft_nd_config_t* cfg);   /* shafft.h:419 */
+int shafftConfig1DInit(shafft_1d_config_t* cfg, size_t globalSize, shafft_t precision,
+                       shafft_comm_t cComm);                          /* shafft.h:435 */
+void shafftConfig1DRelease(shafft_1d_config_t* cfg);                  /* shafft.h:450 */
+int shafftConfig1DResolve(shafft_1d_config_t* cfg);                   /* shafft.h:462 */
+int shafft1DInitFromConfig(void* planPtr, shafft_1d_config_t* cfg);   /* shafft.h:475 */
+/* The plan's communicator: the one-rank handle 0 (shafft.h:489). */
+int shafftGetCommunicator(void* planPtr, shafft_comm_t* outComm);
+
 /* Fortran-facing companions bound by the reference's Fortran module
  * (src/f03/shafftf03.h; src/f03/shafft.f03).  The Fortran communicator
  * handle is read through the pointer and must be 0 (the one-rank handle,

@@ -140,6 +140,18 @@ _SIGS = {
     "shafftGetVersion": (None, [ct.POINTER(ct.c_int)] * 3),
     "shafftGetVersionString": (ct.c_char_p, []),
     "shafftFinalize": (ct.c_int, []),
+    # configuration objects (struct pointers passed as void*)
+    "shafftConfigNDInit": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_size_t), ct.c_int,
+                                      ct.POINTER(ct.c_int), ct.c_int, ct.c_int, ct.c_int, ct.c_size_t,
+                                      ct.c_ssize_t]),
+    "shafftConfigNDRelease": (None, [ct.c_void_p]),
+    "shafftConfigNDResolve": (ct.c_int, [ct.c_void_p]),
+    "shafftNDInitFromConfig": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "shafftConfig1DInit": (ct.c_int, [ct.c_void_p, ct.c_size_t, ct.c_int, ct.c_ssize_t]),
+    "shafftConfig1DRelease": (None, [ct.c_void_p]),
+    "shafftConfig1DResolve": (ct.c_int, [ct.c_void_p]),
+    "shafft1DInitFromConfig": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "shafftGetCommunicator": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_ssize_t)]),
 }
 
 

@@ -7,10 +7,12 @@
 //                      src/cpp/shafft_internal.cpp:584-643 (1D)
 //   1-D factor         src/cpp/fft/hipfft_method/fft1d_method.hip:652-679
 #include <hip/hip_runtime_api.h>
+#include <unistd.h>
 
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -989,5 +991,257 @@ void shafftGetVersion(int* major, int* minor, int* patch) {
 const char* shafftGetVersionString(void) { return "1.1.0"; }
 
 int shafftFinalize(void) { return 0; }
+
+// ---------------------------------------------------------------------------
+// Configuration objects at one rank (shafft.h:203-475; resolution rules of
+// src/cpp/config_info.cpp:200-510 with a communicator of size 1).
+// ---------------------------------------------------------------------------
+
+static const int kConfigApiVersion = 1;
+
+static char* dup_cstr(const char* src, size_t& len) {
+  len = src ? std::strlen(src) : 0;
+  if (!src) return nullptr;
+  char* d = static_cast<char*>(std::malloc(len + 1));
+  if (!d) fail(ERR_ALLOC, "config string");
+  std::memcpy(d, src, len + 1);
+  return d;
+}
+
+extern "C++" {
+// Host name and current HIP device name (none: NULL), config-owned.
+template <typename C>
+static void fill_topology(C* cfg) {
+  char host[256] = {0};
+  if (gethostname(host, sizeof host - 1) != 0) std::strcpy(host, "localhost");
+  std::free(cfg->hostname);
+  cfg->hostname = dup_cstr(host, cfg->hostnameLen);
+  std::free(cfg->deviceName);
+  cfg->deviceName = nullptr;
+  cfg->deviceNameLen = 0;
+  int n = 0, dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceCount(&n) == hipSuccess && n > 0 && hipGetDevice(&dev) == hipSuccess &&
+      hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    cfg->deviceName = dup_cstr(prop.gcnArchName[0] ? prop.gcnArchName : prop.name, cfg->deviceNameLen);
+  (void)hipGetLastError();
+  cfg->nodeId = 0;
+  cfg->nodeCount = 1;
+}
+
+template <typename T>
+static T* alloc_array(int n) {
+  T* a = static_cast<T*>(std::calloc((size_t)n, sizeof(T)));
+  if (!a) fail(ERR_ALLOC, "config array");
+  return a;
+}
+}  // extern "C++"
+
+static void nd_config_free(shafft_nd_config_t* c) {
+  void* owned[] = {c->globalShape,          c->hintCommDims,           c->commDims,
+                   c->initial.subsize,      c->initial.offset,         c->initial.contiguousAxes,
+                   c->initial.distributedAxes, c->output.subsize,      c->output.offset,
+                   c->output.contiguousAxes, c->output.distributedAxes, c->hostname,
+                   c->deviceName};
+  for (void* q : owned) std::free(q);
+  std::memset(c, 0, sizeof *c);
+}
+
+void shafftConfigNDRelease(shafft_nd_config_t* cfg) {
+  if (!cfg || cfg->structSize == 0) return;  // idempotent; safe after a failed Init
+  nd_config_free(cfg);
+}
+
+int shafftConfigNDResolve(shafft_nd_config_t* cfg) {
+  return guarded([&] {
+    if (!cfg) fail(ERR_NULLPTR, "null config");
+    if (cfg->structSize != sizeof(shafft_nd_config_t)) fail(ERR_INVALID_STATE, "config not initialised");
+    if (cfg->ndim < 1) fail(ERR_INVALID_DIM, "ndim must be positive");
+    if (cfg->worldComm != 0) fail(ERR_INVALID_COMM, "communicator must be the one-rank handle 0");
+    const int nd = cfg->ndim;
+    cfg->flags &= ~SHAFFT_CONFIG_RESOLVED;
+    try {
+      for (int i = 0; i < nd; ++i) {
+        if (cfg->globalShape[i] == 0) fail(ERR_INVALID_DIM, "extent must be positive");
+        if (cfg->globalShape[i] > (size_t)INT_MAX) fail(ERR_SIZE_OVERFLOW, "extent exceeds INT_MAX");
+      }
+      if (cfg->precision != SHAFFT_C2C && cfg->precision != SHAFFT_Z2Z)
+        fail(ERR_INVALID_FFTTYPE, "invalid precision");
+    } catch (const Error& e) {
+      cfg->status = e.status;
+      throw;
+    }
+    // One rank: every hint resolves to the grid of ones and no distributed axis.
+    bool grid_changed = false;
+    size_t total = 1;
+    for (int i = 0; i < nd; ++i) {
+      cfg->commDims[i] = 1;
+      grid_changed = grid_changed || cfg->hintCommDims[i] != 1;
+      total *= cfg->globalShape[i];
+    }
+    cfg->nda = 0;
+    cfg->commSize = 1;
+    cfg->allocElements = total;
+    for (shafft_nd_layout_t* l : {&cfg->initial, &cfg->output}) {
+      for (int i = 0; i < nd; ++i) {
+        l->subsize[i] = cfg->globalShape[i];
+        l->offset[i] = 0;
+        l->contiguousAxes[i] = i;
+        l->distributedAxes[i] = 0;  // length nda = 0: unused
+      }
+      l->localElements = total;
+      l->hasLocalElements = total > 0;
+    }
+    cfg->isActive = 1;
+    cfg->activeRank = 0;
+    cfg->activeSize = 1;
+    cfg->activeComm = 0;
+    fill_topology(cfg);
+    cfg->flags = SHAFFT_CONFIG_RESOLVED;
+    if (grid_changed) cfg->flags |= SHAFFT_CONFIG_CHANGED_COMM_DIMS;
+    if (cfg->hintNda != cfg->nda) cfg->flags |= SHAFFT_CONFIG_CHANGED_NDA;
+    cfg->status = 0;
+    return 0;
+  });
+}
+
+int shafftConfigNDInit(shafft_nd_config_t* cfg, int ndim, const size_t* globalShape,
+                       shafft_t precision, const int* commDims, int hintNda,
+                       shafft_decomposition_strategy_t strategy,
+                       shafft_transform_layout_t outputPolicy, size_t memLimit,
+                       shafft_comm_t cComm) {
+  const int rc = guarded([&] {
+    if (!cfg) fail(ERR_NULLPTR, "null config");
+    if (ndim < 1) fail(ERR_INVALID_DIM, "ndim must be positive");
+    if (!globalShape) fail(ERR_NULLPTR, "null globalShape");
+    if (cfg->structSize == sizeof(shafft_nd_config_t)) fail(ERR_INVALID_STATE, "config already initialised");
+    if (cComm != 0) fail(ERR_INVALID_COMM, "communicator must be the one-rank handle 0");
+    if (outputPolicy != SHAFFT_LAYOUT_REDISTRIBUTED && outputPolicy != SHAFFT_LAYOUT_INITIAL)
+      fail(ERR_INVALID_LAYOUT, "invalid output policy");
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->structSize = sizeof(shafft_nd_config_t);
+    cfg->apiVersion = kConfigApiVersion;
+    cfg->ndim = ndim;
+    cfg->activeRank = -1;
+    try {
+      cfg->globalShape = alloc_array<size_t>(ndim);
+      cfg->hintCommDims = alloc_array<int>(ndim);
+      cfg->commDims = alloc_array<int>(ndim);
+      for (shafft_nd_layout_t* l : {&cfg->initial, &cfg->output}) {
+        l->subsize = alloc_array<size_t>(ndim);
+        l->offset = alloc_array<size_t>(ndim);
+        l->contiguousAxes = alloc_array<int>(ndim);
+        l->distributedAxes = alloc_array<int>(ndim);
+      }
+    } catch (...) {
+      nd_config_free(cfg);
+      throw;
+    }
+    for (int i = 0; i < ndim; ++i) {
+      cfg->globalShape[i] = globalShape[i];
+      cfg->hintCommDims[i] = commDims ? commDims[i] : 0;  // 0: no preference
+    }
+    cfg->precision = precision;
+    cfg->hintNda = hintNda;
+    cfg->strategy = strategy;
+    cfg->outputPolicy = outputPolicy;
+    cfg->memLimit = memLimit;
+    cfg->worldComm = 0;
+    cfg->activeComm = 0;
+    return 0;
+  });
+  if (rc != 0) return rc;
+  const int r2 = shafftConfigNDResolve(cfg);
+  if (r2 != 0) nd_config_free(cfg);  // as a failed Init: nothing left to release
+  return r2;
+}
+
+int shafftNDInitFromConfig(void* planPtr, shafft_nd_config_t* cfg) {
+  if (!cfg) return guarded([&]() -> int { fail(ERR_NULLPTR, "null config"); });
+  if (!(cfg->flags & SHAFFT_CONFIG_RESOLVED)) {
+    const int rc = shafftConfigNDResolve(cfg);
+    if (rc != 0) return rc;
+  }
+  std::vector<int> dims(cfg->ndim), grid(cfg->ndim);
+  for (int i = 0; i < cfg->ndim; ++i) {
+    dims[i] = (int)cfg->globalShape[i];
+    grid[i] = cfg->commDims[i];
+  }
+  return shafftNDInit(planPtr, cfg->ndim, grid.data(), dims.data(), cfg->precision,
+                      cfg->worldComm, cfg->outputPolicy);
+}
+
+void shafftConfig1DRelease(shafft_1d_config_t* cfg) {
+  if (!cfg || cfg->structSize == 0) return;
+  std::free(cfg->hostname);
+  std::free(cfg->deviceName);
+  std::memset(cfg, 0, sizeof *cfg);
+}
+
+int shafftConfig1DResolve(shafft_1d_config_t* cfg) {
+  return guarded([&] {
+    if (!cfg) fail(ERR_NULLPTR, "null config");
+    if (cfg->structSize != sizeof(shafft_1d_config_t)) fail(ERR_INVALID_STATE, "config not initialised");
+    if (cfg->globalSize == 0) fail(ERR_INVALID_DIM, "globalSize must be positive");
+    if (cfg->worldComm != 0) fail(ERR_INVALID_COMM, "communicator must be the one-rank handle 0");
+    cfg->flags &= ~SHAFFT_CONFIG_RESOLVED;
+    for (shafft_1d_layout_t* l : {&cfg->initial, &cfg->output}) {
+      l->localSize = cfg->globalSize;
+      l->localStart = 0;
+      l->hasLocalElements = 1;
+    }
+    cfg->allocElements = cfg->globalSize;
+    cfg->isActive = 1;
+    cfg->activeRank = 0;
+    cfg->activeSize = 1;
+    cfg->activeComm = 0;
+    fill_topology(cfg);
+    cfg->flags = SHAFFT_CONFIG_RESOLVED;
+    cfg->status = 0;
+    return 0;
+  });
+}
+
+int shafftConfig1DInit(shafft_1d_config_t* cfg, size_t globalSize, shafft_t precision,
+                       shafft_comm_t cComm) {
+  const int rc = guarded([&] {
+    if (!cfg) fail(ERR_NULLPTR, "null config");
+    if (globalSize == 0) fail(ERR_INVALID_DIM, "globalSize must be positive");
+    if (precision != SHAFFT_C2C && precision != SHAFFT_Z2Z) fail(ERR_INVALID_FFTTYPE, "invalid precision");
+    if (cfg->structSize == sizeof(shafft_1d_config_t)) fail(ERR_INVALID_STATE, "config already initialised");
+    if (cComm != 0) fail(ERR_INVALID_COMM, "communicator must be the one-rank handle 0");
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->structSize = sizeof(shafft_1d_config_t);
+    cfg->apiVersion = kConfigApiVersion;
+    cfg->globalSize = globalSize;
+    cfg->precision = precision;
+    cfg->activeRank = -1;
+    return 0;
+  });
+  if (rc != 0) return rc;
+  const int r2 = shafftConfig1DResolve(cfg);
+  if (r2 != 0) shafftConfig1DRelease(cfg);
+  return r2;
+}
+
+int shafft1DInitFromConfig(void* planPtr, shafft_1d_config_t* cfg) {
+  if (!cfg) return guarded([&]() -> int { fail(ERR_NULLPTR, "null config"); });
+  if (!(cfg->flags & SHAFFT_CONFIG_RESOLVED)) {
+    const int rc = shafftConfig1DResolve(cfg);
+    if (rc != 0) return rc;
+  }
+  return shafft1DInit(planPtr, cfg->globalSize, cfg->initial.localSize, cfg->initial.localStart,
+                      cfg->precision, cfg->worldComm);
+}
+
+int shafftGetCommunicator(void* planPtr, shafft_comm_t* outComm) {
+  return guarded([&] {
+    if (!outComm) fail(ERR_NULLPTR, "null output");
+    Plan* p = as_plan(planPtr);
+    if (p->state == State::UNINITIALIZED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
+    *outComm = 0;
+    return 0;
+  });
+}
 
 }  // extern "C"

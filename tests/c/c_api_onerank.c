@@ -167,6 +167,119 @@ static void test_configuration(void) {
   check("Configuration1D at one rank", rc == 0 && ln == 1024 && ls == 0 && la >= 1024);
 }
 
+/* Configuration objects (shafft.h:203-475) at one rank: resolved fields,
+ * flags, error cases, release. */
+static void test_config_objects_host(void) {
+  shafft_nd_config_t c;
+  memset(&c, 0, sizeof c);
+  const size_t shape[3] = {64, 48, 30};
+  const int hint[3] = {0, 0, 0};
+  int rc = shafftConfigNDInit(&c, 3, shape, SHAFFT_Z2Z, hint, 0, SHAFFT_MAXIMIZE_NDA,
+                              SHAFFT_LAYOUT_REDISTRIBUTED, 0, 0);
+  int ok = rc == 0 && c.structSize == sizeof c && c.ndim == 3 && c.nda == 0 && c.commSize == 1 &&
+           c.allocElements == 64 * 48 * 30 && c.isActive == 1 && c.activeRank == 0 &&
+           c.activeSize == 1 && c.nodeCount == 1 && c.hostname != NULL &&
+           c.hostnameLen == strlen(c.hostname) && (c.flags & SHAFFT_CONFIG_RESOLVED) &&
+           (c.flags & SHAFFT_CONFIG_CHANGED_COMM_DIMS) && !(c.flags & SHAFFT_CONFIG_CHANGED_NDA) &&
+           c.status == 0;
+  for (int i = 0; ok && i < 3; ++i)
+    ok = c.commDims[i] == 1 && c.initial.subsize[i] == shape[i] && c.initial.offset[i] == 0 &&
+         c.output.subsize[i] == shape[i] && c.initial.contiguousAxes[i] == i &&
+         c.globalShape[i] == shape[i];
+  ok = ok && c.initial.localElements == c.allocElements && c.initial.hasLocalElements == 1;
+  check("ConfigNDInit resolves the one-rank layout", ok);
+  check("ConfigNDInit twice -> INVALID_STATE",
+        shafftConfigNDInit(&c, 3, shape, SHAFFT_Z2Z, hint, 0, SHAFFT_MAXIMIZE_NDA,
+                           SHAFFT_LAYOUT_REDISTRIBUTED, 0, 0) == SHAFFT_ERR_INVALID_STATE);
+  check("ConfigNDResolve again", shafftConfigNDResolve(&c) == 0 && (c.flags & SHAFFT_CONFIG_RESOLVED));
+  shafftConfigNDRelease(&c);
+  check("ConfigNDRelease zero-fills", c.structSize == 0 && c.globalShape == NULL && c.hostname == NULL);
+  shafftConfigNDRelease(&c); /* idempotent */
+  check("ConfigNDResolve after release -> INVALID_STATE",
+        shafftConfigNDResolve(&c) == SHAFFT_ERR_INVALID_STATE);
+
+  const int ones[3] = {1, 1, 1};
+  memset(&c, 0, sizeof c);
+  rc = shafftConfigNDInit(&c, 3, shape, SHAFFT_C2C, ones, 0, SHAFFT_MINIMIZE_NDA,
+                          SHAFFT_LAYOUT_INITIAL, 0, 0);
+  check("ConfigNDInit with the grid of ones: no CHANGED flags",
+        rc == 0 && c.flags == SHAFFT_CONFIG_RESOLVED && c.outputPolicy == SHAFFT_LAYOUT_INITIAL);
+  shafftConfigNDRelease(&c);
+
+  const size_t bad[2] = {16, 0};
+  memset(&c, 0, sizeof c);
+  check("ConfigNDInit zero extent -> INVALID_DIM, released",
+        shafftConfigNDInit(&c, 2, bad, SHAFFT_C2C, NULL, 0, SHAFFT_MAXIMIZE_NDA,
+                           SHAFFT_LAYOUT_REDISTRIBUTED, 0, 0) == SHAFFT_ERR_INVALID_DIM &&
+            c.structSize == 0);
+  check("ConfigNDInit comm != 0 -> INVALID_COMM",
+        shafftConfigNDInit(&c, 3, shape, SHAFFT_C2C, NULL, 0, SHAFFT_MAXIMIZE_NDA,
+                           SHAFFT_LAYOUT_REDISTRIBUTED, 0, 7) == SHAFFT_ERR_INVALID_COMM);
+
+  shafft_1d_config_t d;
+  memset(&d, 0, sizeof d);
+  rc = shafftConfig1DInit(&d, 1000003, SHAFFT_C2C, 0);
+  check("Config1DInit resolves the one-rank layout",
+        rc == 0 && d.structSize == sizeof d && d.globalSize == 1000003 &&
+            d.initial.localSize == 1000003 && d.initial.localStart == 0 &&
+            d.output.localSize == 1000003 && d.allocElements == 1000003 && d.isActive == 1 &&
+            d.flags == SHAFFT_CONFIG_RESOLVED);
+  shafftConfig1DRelease(&d);
+  check("Config1DRelease zero-fills", d.structSize == 0 && d.hostname == NULL);
+  check("Config1DInit size 0 -> INVALID_DIM", shafftConfig1DInit(&d, 0, SHAFFT_C2C, 0) == SHAFFT_ERR_INVALID_DIM);
+
+  void* plan = NULL;
+  shafft_comm_t comm = 99;
+  shafftNDCreate(&plan);
+  check("GetCommunicator before init -> PLAN_NOT_INIT",
+        shafftGetCommunicator(plan, &comm) == SHAFFT_ERR_PLAN_NOT_INIT);
+  memset(&c, 0, sizeof c);
+  shafftConfigNDInit(&c, 3, shape, SHAFFT_C2C, NULL, 0, SHAFFT_MAXIMIZE_NDA,
+                     SHAFFT_LAYOUT_REDISTRIBUTED, 0, 0);
+  size_t alloc = 0, g = 0;
+  rc = shafftNDInitFromConfig(plan, &c);
+  check("NDInitFromConfig configures the plan",
+        rc == 0 && shafftGetAllocSize(plan, &alloc) == 0 && alloc == c.allocElements &&
+            shafftGetGlobalSize(plan, &g) == 0 && g == c.allocElements);
+  check("GetCommunicator -> the one-rank handle", shafftGetCommunicator(plan, &comm) == 0 && comm == 0);
+  shafftConfigNDRelease(&c);
+  shafftDestroy(&plan);
+}
+
+/* Plans made from config objects run the same round trips. */
+static void test_config_objects_gpu(void) {
+  shafft_nd_config_t c;
+  memset(&c, 0, sizeof c);
+  const size_t shape[3] = {24, 40, 18};
+  void *plan = NULL, *data = NULL, *work = NULL, *d = NULL, *w = NULL;
+  int rc = shafftConfigNDInit(&c, 3, shape, SHAFFT_Z2Z, NULL, 0, SHAFFT_MAXIMIZE_NDA,
+                              SHAFFT_LAYOUT_REDISTRIBUTED, 0, 0);
+  rc |= shafftNDCreate(&plan) | shafftNDInitFromConfig(plan, &c);
+  rc |= shafftAllocBufferD(c.allocElements, &data) | shafftAllocBufferD(c.allocElements, &work);
+  rc |= shafftSetBuffers(plan, data, work) | shafftPlan(plan);
+  const size_t n = c.allocElements;
+  double* x = (double*)malloc(2 * n * sizeof(double));
+  double* y = (double*)malloc(2 * n * sizeof(double));
+  for (size_t i = 0; i < 2 * n; ++i) x[i] = next_uniform();
+  rc |= shafftCopyToBufferD(data, x, n);
+  rc |= shafftExecute(plan, SHAFFT_FORWARD) | shafftNormalize(plan);
+  rc |= shafftExecute(plan, SHAFFT_BACKWARD) | shafftNormalize(plan);
+  rc |= shafftGetBuffers(plan, &d, &w) | shafftCopyFromBufferD(y, d, n);
+  double num = 0, den = 0;
+  for (size_t i = 0; i < 2 * n; ++i) {
+    num += (y[i] - x[i]) * (y[i] - x[i]);
+    den += x[i] * x[i];
+  }
+  printf("  config-object N-D round trip rel L2 %.3e\n", sqrt(num / den));
+  check("N-D round trip of a plan from a config object", rc == 0 && sqrt(num / den) <= 1e-12);
+  free(x);
+  free(y);
+  shafftFreeBufferD(data);
+  shafftFreeBufferD(work);
+  shafftDestroy(&plan);
+  shafftConfigNDRelease(&c);
+}
+
 /* ------------------------------------------------------ N-D transforms */
 
 typedef struct {
@@ -395,12 +508,14 @@ int main(int argc, char** argv) {
   test_life_cycle();
   test_configuration();
   test_alloc_edges_host();
+  test_config_objects_host();
   if (host_only) {
     printf("%d check(s) failed\n", g_fail);
     return g_fail;
   }
   test_buffers();
   test_alloc_too_large();
+  test_config_objects_gpu();
   {
     int d1[3] = {32, 32, 32};
     test_nd_roundtrip(3, d1, SHAFFT_C2C, "N-D round trip 32x32x32 complex64");
