@@ -393,6 +393,80 @@ int shafftConfiguration1Df03(size_t N, size_t* localN, size_t* localStart,
                              shafft_fint_t* fHandle);                 /* shafftf03.h:32 */
 int shafft1DInitf03(void* planPtr, size_t N, size_t localN, size_t localStart,
                     shafft_t precision, shafft_fint_t* fHandle);      /* shafftf03.h:39 */
+/* Config structs on the heap for the Fortran derived types (shafftf03.h:47-50). */
+shafft_nd_config_t* shafftConfigNDAllocf03(void);
+void shafftConfigNDFreef03(shafft_nd_config_t* cfg);
+shafft_1d_config_t* shafftConfig1DAllocf03(void);
+void shafftConfig1DFreef03(shafft_1d_config_t* cfg);
+/* Config wrappers taking the Fortran handle (shafftf03.h:53-71). */
+int shafftConfigNDInitf03(shafft_nd_config_t* cfg, int ndim, const size_t* globalShape,
+                          shafft_t precision, const int* commDims, int hintNda,
+                          shafft_decomposition_strategy_t strategy,
+                          shafft_transform_layout_t outputPolicy, size_t memLimit,
+                          shafft_fint_t* fComm);
+int shafftConfigNDResolvef03(shafft_nd_config_t* cfg);
+int shafftNDInitFromConfigf03(void* planPtr, shafft_nd_config_t* cfg);
+int shafftConfig1DInitf03(shafft_1d_config_t* cfg, size_t globalSize, shafft_t precision,
+                          shafft_fint_t* fComm);
+int shafftConfig1DResolvef03(shafft_1d_config_t* cfg);
+int shafft1DInitFromConfigf03(void* planPtr, shafft_1d_config_t* cfg);
+int shafftGetCommunicatorf03(void* planPtr, shafft_fint_t* outFint);
+/* Per-field accessors behind the Fortran derived-type sync
+ * (shafftf03.h:76-146); a null config reads as 0 and writes nothing. */
+int shafftConfigNDGetNdim(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetPrecision(const shafft_nd_config_t* cfg);
+size_t shafftConfigNDGetAllocElements(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetIsActive(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetActiveRank(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetActiveSize(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetNda(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetCommSize(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetStatus(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetOutputPolicy(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetStrategy(const shafft_nd_config_t* cfg);
+size_t shafftConfigNDGetMemLimit(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetHintNda(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetFlags(const shafft_nd_config_t* cfg);
+shafft_fint_t shafftConfigNDGetWorldComm(const shafft_nd_config_t* cfg);
+shafft_fint_t shafftConfigNDGetActiveComm(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetNodeId(const shafft_nd_config_t* cfg);
+int shafftConfigNDGetNodeCount(const shafft_nd_config_t* cfg);
+size_t shafftConfigNDGetHostnameLen(const shafft_nd_config_t* cfg);
+void shafftConfigNDGetHostname(const shafft_nd_config_t* cfg, char* out, size_t maxLen);
+size_t shafftConfigNDGetDeviceNameLen(const shafft_nd_config_t* cfg);
+void shafftConfigNDGetDeviceName(const shafft_nd_config_t* cfg, char* out, size_t maxLen);
+void shafftConfigNDGetGlobalShape(const shafft_nd_config_t* cfg, size_t* out);
+void shafftConfigNDGetCommDims(const shafft_nd_config_t* cfg, int* out);
+void shafftConfigNDGetHintCommDims(const shafft_nd_config_t* cfg, int* out);
+void shafftConfigNDGetInitialSubsize(const shafft_nd_config_t* cfg, size_t* out);
+void shafftConfigNDGetInitialOffset(const shafft_nd_config_t* cfg, size_t* out);
+void shafftConfigNDGetOutputSubsize(const shafft_nd_config_t* cfg, size_t* out);
+void shafftConfigNDGetOutputOffset(const shafft_nd_config_t* cfg, size_t* out);
+void shafftConfigNDSetOutputPolicy(shafft_nd_config_t* cfg, int pol);
+void shafftConfigNDSetStrategy(shafft_nd_config_t* cfg, int s);
+void shafftConfigNDSetMemLimit(shafft_nd_config_t* cfg, size_t limit);
+void shafftConfigNDSetHintNda(shafft_nd_config_t* cfg, int nda);
+void shafftConfigNDSetHintCommDims(shafft_nd_config_t* cfg, const int* dims);
+size_t shafftConfig1DGetGlobalSize(const shafft_1d_config_t* cfg);
+int shafftConfig1DGetPrecision(const shafft_1d_config_t* cfg);
+size_t shafftConfig1DGetAllocElements(const shafft_1d_config_t* cfg);
+int shafftConfig1DGetIsActive(const shafft_1d_config_t* cfg);
+int shafftConfig1DGetActiveRank(const shafft_1d_config_t* cfg);
+int shafftConfig1DGetActiveSize(const shafft_1d_config_t* cfg);
+int shafftConfig1DGetStatus(const shafft_1d_config_t* cfg);
+int shafftConfig1DGetFlags(const shafft_1d_config_t* cfg);
+shafft_fint_t shafftConfig1DGetWorldComm(const shafft_1d_config_t* cfg);
+shafft_fint_t shafftConfig1DGetActiveComm(const shafft_1d_config_t* cfg);
+int shafftConfig1DGetNodeId(const shafft_1d_config_t* cfg);
+int shafftConfig1DGetNodeCount(const shafft_1d_config_t* cfg);
+size_t shafftConfig1DGetHostnameLen(const shafft_1d_config_t* cfg);
+void shafftConfig1DGetHostname(const shafft_1d_config_t* cfg, char* out, size_t maxLen);
+size_t shafftConfig1DGetDeviceNameLen(const shafft_1d_config_t* cfg);
+void shafftConfig1DGetDeviceName(const shafft_1d_config_t* cfg, char* out, size_t maxLen);
+size_t shafftConfig1DGetInitialLocalSize(const shafft_1d_config_t* cfg);
+size_t shafftConfig1DGetInitialLocalStart(const shafft_1d_config_t* cfg);
+size_t shafftConfig1DGetOutputLocalSize(const shafft_1d_config_t* cfg);
+size_t shafftConfig1DGetOutputLocalStart(const shafft_1d_config_t* cfg);
 
 #ifdef __cplusplus
 } /* extern "C" */

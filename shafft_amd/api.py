@@ -154,6 +154,45 @@ _SIGS = {
     "shafftGetCommunicator": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_ssize_t)]),
 }
 
+# Fortran-side config companions and per-field accessors (config pointers as
+# void*): name -> (restype, argtypes).
+_VP, _SZ = ct.c_void_p, ct.c_size_t
+_SIGS.update({
+    "shafftConfigNDAllocf03": (_VP, []),
+    "shafftConfigNDFreef03": (None, [_VP]),
+    "shafftConfig1DAllocf03": (_VP, []),
+    "shafftConfig1DFreef03": (None, [_VP]),
+    "shafftConfigNDInitf03": (ct.c_int, [_VP, ct.c_int, ct.POINTER(_SZ), ct.c_int, ct.POINTER(ct.c_int),
+                                         ct.c_int, ct.c_int, ct.c_int, _SZ, ct.POINTER(ct.c_int)]),
+    "shafftConfigNDResolvef03": (ct.c_int, [_VP]),
+    "shafftNDInitFromConfigf03": (ct.c_int, [_VP, _VP]),
+    "shafftConfig1DInitf03": (ct.c_int, [_VP, _SZ, ct.c_int, ct.POINTER(ct.c_int)]),
+    "shafftConfig1DResolvef03": (ct.c_int, [_VP]),
+    "shafft1DInitFromConfigf03": (ct.c_int, [_VP, _VP]),
+    "shafftGetCommunicatorf03": (ct.c_int, [_VP, ct.POINTER(ct.c_int)]),
+})
+for _n in ("Ndim", "Precision", "IsActive", "ActiveRank", "ActiveSize", "Nda", "CommSize", "Status",
+           "OutputPolicy", "Strategy", "HintNda", "Flags", "WorldComm", "ActiveComm", "NodeId", "NodeCount"):
+    _SIGS["shafftConfigNDGet" + _n] = (ct.c_int, [_VP])
+for _n in ("AllocElements", "MemLimit", "HostnameLen", "DeviceNameLen"):
+    _SIGS["shafftConfigNDGet" + _n] = (_SZ, [_VP])
+for _n in ("Hostname", "DeviceName"):
+    _SIGS["shafftConfigNDGet" + _n] = (None, [_VP, ct.c_char_p, _SZ])
+    _SIGS["shafftConfig1DGet" + _n] = (None, [_VP, ct.c_char_p, _SZ])
+for _n in ("GlobalShape", "InitialSubsize", "InitialOffset", "OutputSubsize", "OutputOffset"):
+    _SIGS["shafftConfigNDGet" + _n] = (None, [_VP, ct.POINTER(_SZ)])
+for _n in ("CommDims", "HintCommDims"):
+    _SIGS["shafftConfigNDGet" + _n] = (None, [_VP, ct.POINTER(ct.c_int)])
+for _n, _t in (("OutputPolicy", ct.c_int), ("Strategy", ct.c_int), ("MemLimit", _SZ), ("HintNda", ct.c_int),
+               ("HintCommDims", ct.POINTER(ct.c_int))):
+    _SIGS["shafftConfigNDSet" + _n] = (None, [_VP, _t])
+for _n in ("Precision", "IsActive", "ActiveRank", "ActiveSize", "Status", "Flags", "WorldComm",
+           "ActiveComm", "NodeId", "NodeCount"):
+    _SIGS["shafftConfig1DGet" + _n] = (ct.c_int, [_VP])
+for _n in ("GlobalSize", "AllocElements", "HostnameLen", "DeviceNameLen", "InitialLocalSize",
+           "InitialLocalStart", "OutputLocalSize", "OutputLocalStart"):
+    _SIGS["shafftConfig1DGet" + _n] = (_SZ, [_VP])
+
 
 def bind(path: str):
     """Load a library exposing the shafft_amd C ABI and type its symbols."""

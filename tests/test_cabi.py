@@ -45,3 +45,68 @@ def test_host_only_calls():
     buf = ct.create_string_buffer(256)
     assert L.shafftLastErrorMessage(buf, 256) > 0
     assert L.shafftDestroy(ct.byref(h)) == 0
+
+
+def test_fortran_config_accessors():
+    """The Fortran module's config path (src/f03/shafftf03.h:46-146) at one
+    rank: heap config, Init through the Fortran handle, field accessors,
+    plan from the config, and the handle check.  Host-only calls."""
+    L = api.bind(api.library_path())
+    cfg = L.shafftConfigNDAllocf03()
+    assert cfg
+    shape = (ct.c_size_t * 3)(40, 30, 20)
+    zero = ct.c_int(0)
+    assert L.shafftConfigNDInitf03(cfg, 3, shape, api.Z2Z, None, 0, 0, 1, 0, ct.byref(zero)) == 0
+    assert L.shafftConfigNDGetNdim(cfg) == 3 and L.shafftConfigNDGetPrecision(cfg) == api.Z2Z
+    assert L.shafftConfigNDGetAllocElements(cfg) == 40 * 30 * 20
+    assert L.shafftConfigNDGetNda(cfg) == 0 and L.shafftConfigNDGetCommSize(cfg) == 1
+    assert L.shafftConfigNDGetIsActive(cfg) == 1 and L.shafftConfigNDGetActiveRank(cfg) == 0
+    assert L.shafftConfigNDGetOutputPolicy(cfg) == 1 and L.shafftConfigNDGetWorldComm(cfg) == 0
+    assert L.shafftConfigNDGetFlags(cfg) & 8  # RESOLVED
+    sub, off, grid = (ct.c_size_t * 3)(), (ct.c_size_t * 3)(9, 9, 9), (ct.c_int * 3)()
+    L.shafftConfigNDGetInitialSubsize(cfg, sub)
+    L.shafftConfigNDGetOutputOffset(cfg, off)
+    L.shafftConfigNDGetCommDims(cfg, grid)
+    assert list(sub) == [40, 30, 20] and list(off) == [0, 0, 0] and list(grid) == [1, 1, 1]
+    n = L.shafftConfigNDGetHostnameLen(cfg)
+    buf = ct.create_string_buffer(n + 1)
+    L.shafftConfigNDGetHostname(cfg, buf, n + 1)
+    assert n > 0 and len(buf.value) == n
+    L.shafftConfigNDSetHintNda(cfg, 1)
+    assert L.shafftConfigNDResolvef03(cfg) == 0 and L.shafftConfigNDGetFlags(cfg) & 2  # CHANGED_NDA
+    plan = ct.c_void_p()
+    assert L.shafftNDCreate(ct.byref(plan)) == 0
+    assert L.shafftNDInitFromConfigf03(plan, cfg) == 0
+    alloc = ct.c_size_t()
+    assert L.shafftGetAllocSize(plan, ct.byref(alloc)) == 0 and alloc.value == 40 * 30 * 20
+    fcomm = ct.c_int(5)
+    assert L.shafftGetCommunicatorf03(plan, ct.byref(fcomm)) == 0 and fcomm.value == 0
+    assert L.shafftDestroy(ct.byref(plan)) == 0
+    L.shafftConfigNDRelease(cfg)
+    assert L.shafftConfigNDGetNdim(cfg) == 0
+    bad = ct.c_int(3)
+    assert L.shafftConfigNDInitf03(cfg, 3, shape, api.C2C, None, 0, 0, 0, 0, ct.byref(bad)) == 2  # INVALID_COMM
+    L.shafftConfigNDFreef03(cfg)
+
+    c1 = L.shafftConfig1DAllocf03()
+    assert L.shafftConfig1DInitf03(c1, 1000003, api.C2C, ct.byref(zero)) == 0
+    assert L.shafftConfig1DGetGlobalSize(c1) == 1000003 and L.shafftConfig1DGetAllocElements(c1) == 1000003
+    assert L.shafftConfig1DGetInitialLocalSize(c1) == 1000003 and L.shafftConfig1DGetOutputLocalStart(c1) == 0
+    p1 = ct.c_void_p()
+    assert L.shafft1DCreate(ct.byref(p1)) == 0 and L.shafft1DInitFromConfigf03(p1, c1) == 0
+    assert L.shafftDestroy(ct.byref(p1)) == 0
+    L.shafftConfig1DRelease(c1)
+    L.shafftConfig1DFreef03(c1)
+    assert L.shafftConfigNDGetNdim(None) == 0  # null config reads as 0
+
+
+def test_every_fortran_bound_symbol_is_exported():
+    """Every C symbol the reference's Fortran module binds (bind(C, name=...)
+    in src/f03/shafft.f03; list in tests/golden/f03_bound_symbols.txt, made
+    by tests/golden/make_f03_symbols.py) is exported, so the module links
+    against this library."""
+    L = ct.CDLL(api.library_path())
+    names = open(os.path.join(ROOT, "tests", "golden", "f03_bound_symbols.txt")).read().split()
+    assert len(names) > 90
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
