@@ -238,10 +238,9 @@ class FFT1D : public FFT {
   // ca = {}, da = {0}: the single axis is the (block-)distributed one.
   [[nodiscard]] int getAxes(std::vector<int>& ca, std::vector<int>& da,
                             TensorLayout layout = TensorLayout::CURRENT) const noexcept {
-    (void)layout;
     ca.clear();
     da.assign(1, 0);
-    return h_.get() ? 0 : SHAFFT_ERR_NULLPTR;
+    return shafftGetAxes(h_.get(), nullptr, da.data(), detail::c_layout(layout));
   }
 };
 

@@ -801,13 +801,16 @@ int shafftGetAxes(void* planPtr, int* ca, int* da, shafft_tensor_layout_t layout
   return guarded([&] {
     Plan* p = as_plan(planPtr);
     if (p->state == State::UNINITIALIZED) fail(ERR_PLAN_NOT_INIT, "plan not initialised");
-    (void)da;
     if (layout < SHAFFT_TENSOR_LAYOUT_CURRENT || layout > SHAFFT_TENSOR_LAYOUT_REDISTRIBUTED)
       fail(ERR_INVALID_LAYOUT, "invalid layout");
     if (p->kind == Kind::ND) {
       if (!ca) fail(ERR_NULLPTR, "null ca");
       // One rank: every axis is contiguous, none distributed (slab_nd.cpp:522-526).
       for (size_t i = 0; i < p->dims.size(); ++i) ca[i] = (int)i;
+    } else if (da) {
+      // 1-D: no contiguous axis, the single axis is the distributed one
+      // (src/cpp/shafft.cpp:776-786: nca 0, nda 1).
+      da[0] = 0;
     }
     return 0;
   });
