@@ -62,6 +62,19 @@ static void host_checks() {
                                                    shafft::DecompositionStrategy::MAXIMIZE_NDA, 0,
                                                    shafft::kOneRank) == 0 &&
                                nda == 0 && cs == 1 && sub == std::vector<size_t>{64, 64, 32});
+  shafft_nd_config_t cfg;
+  std::memset(&cfg, 0, sizeof cfg);
+  const size_t shape[2] = {96, 210};
+  shafft::FFTND fromcfg;
+  check("FFTND init from a config object",
+        shafftConfigNDInit(&cfg, 2, shape, SHAFFT_Z2Z, nullptr, 0, SHAFFT_MAXIMIZE_NDA,
+                           SHAFFT_LAYOUT_REDISTRIBUTED, 0, shafft::kOneRank) == 0 &&
+            fromcfg.init(cfg) == 0 && fromcfg.allocSize() == 96 * 210 && fromcfg.ndim() == 2 &&
+            fromcfg.fftType() == shafft::FFTType::Z2Z);
+  shafftConfigNDRelease(&cfg);
+  shafft::Comm comm = 7;
+  check("getCommunicator", fromcfg.getCommunicator(&comm) == 0 && comm == shafft::kOneRank);
+
   const shafft::Version v = shafft::getVersion();
   check("version and backend", v.major >= 1 && std::strchr(shafft::getVersionString(), '.') &&
                                    std::strlen(shafft::getBackendName()) > 0);
