@@ -111,6 +111,15 @@ int shafftAmd1DExecutePass(shafft_amd_1d_plan* plan, int pass, void* in, int64_t
 int shafftAmdNDDescribe(shafft_amd_nd_plan* plan, char* buf, int buflen);
 int shafftAmd1DDescribe(shafft_amd_1d_plan* plan, char* buf, int buflen);
 
+/* Synthetic input of the benchmark workloads into host memory: `count`
+ * interleaved complex values (precision 0 float, 1 double) from
+ * std::mt19937_64(seed); distribution 0 = re, im uniform in [-1, 1) as
+ * 2 ((x >> 11) 2^-53) - 1, 1 = Box-Muller Gaussian per element.  No reference
+ * counterpart (the reference's tests draw their inputs from std::mt19937_64,
+ * tests/integration/cpp/test_correctness.cpp:382). */
+int shafftAmdSyntheticInput(void* host_out, int64_t count, uint64_t seed, int distribution,
+                            int precision);
+
 /* Batched uneven transforms (no reference counterpart: the reference runs
  * one plan per shape).  `count` independent transforms, transform t having
  * ndims[t] axes taken in order from dims (row-major, last axis fastest),

@@ -6,6 +6,6 @@ benchmark's CPU-baseline leg as the timed CPU port.  Nothing in the product
 restates and which reference lines it follows.
 """
 from .oracle import (  # noqa: F401
-    dftn, fftn_pow2, uniform_input, norm_factor_nd, norm_factor_1d, rel_l2,
+    dftn, fftn_pow2, uniform_input, gaussian_input, norm_factor_nd, norm_factor_1d, rel_l2,
     check_rel_error, threads, load,
 )

@@ -17,6 +17,10 @@
  *   double for power-of-two n (CPU baseline; pinned against the direct DFT
  *   by the tests), with OpenMP over lines.
  *
+ * oracle_uniform / oracle_gaussian: the synthetic input recipe
+ *   (std::mt19937_64 seeded per workload, uniform [-1,1) or Box-Muller
+ *   Gaussian re/im per element).
+ *
  * Complex data is interleaved (re, im) doubles.
  */
 #include <math.h>
@@ -193,6 +197,22 @@ void oracle_uniform(double* out, int64_t count, uint64_t seed) {
   for (int64_t i = 0; i < 2 * count; ++i) {
     const double u = (double)(mt64_next(&s) >> 11) * (1.0 / 9007199254740992.0);
     out[i] = 2.0 * u - 1.0;
+  }
+}
+
+/* Gaussian input recipe (Box-Muller on consecutive draws): per element
+ * u1 = ((x0 >> 11) + 1) * 2^-53 in (0, 1], u2 = (x1 >> 11) * 2^-53 in
+ * [0, 1), r = sqrt(-2 ln u1), re = r cos(2 pi u2), im = r sin(2 pi u2). */
+void oracle_gaussian(double* out, int64_t count, uint64_t seed) {
+  mt64_t s;
+  mt64_seed(&s, seed);
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int64_t i = 0; i < count; ++i) {
+    const double u1 = (double)((mt64_next(&s) >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(mt64_next(&s) >> 11) * (1.0 / 9007199254740992.0);
+    const double r = sqrt(-2.0 * log(u1));
+    out[2 * i] = r * cos(two_pi * u2);
+    out[2 * i + 1] = r * sin(two_pi * u2);
   }
 }
 

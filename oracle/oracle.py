@@ -36,6 +36,8 @@ def load():
         L.oracle_uniform.restype = None
         L.oracle_mt64_raw.argtypes = [ct.c_void_p, ct.c_int64, ct.c_uint64]
         L.oracle_mt64_raw.restype = None
+        L.oracle_gaussian.argtypes = [ct.c_void_p, ct.c_int64, ct.c_uint64]
+        L.oracle_gaussian.restype = None
         _LIB = L
     return _LIB
 
@@ -83,7 +85,18 @@ def uniform_input(shape, seed: int, dtype=np.complex128) -> np.ndarray:
     n = int(np.prod(shape))
     out = np.empty(n, dtype=np.complex128)
     load().oracle_uniform(out.ctypes.data, n, seed)
-    return out.reshape(shape).astype(dtype)
+    return out.reshape(shape).astype(dtype, copy=False)
+
+
+def gaussian_input(shape, seed: int, dtype=np.complex128) -> np.ndarray:
+    """std::mt19937_64(seed), Box-Muller on consecutive draws per element:
+    u1 = ((x0>>11)+1)*2^-53 in (0,1], u2 = (x1>>11)*2^-53, r = sqrt(-2 ln u1),
+    value r*(cos 2 pi u2 + i sin 2 pi u2).  complex64 inputs are the FP64
+    draws rounded to float."""
+    n = int(np.prod(shape))
+    out = np.empty(n, dtype=np.complex128)
+    load().oracle_gaussian(out.ctypes.data, n, seed)
+    return out.reshape(shape).astype(dtype, copy=False)
 
 
 def mt64_raw(count: int, seed: int) -> np.ndarray:

@@ -63,6 +63,7 @@ _SIGS = {
     "shafftAmd1DPassBytes": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64)]),
     "shafftAmdNDDescribe": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_int]),
     "shafftAmd1DDescribe": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_int]),
+    "shafftAmdSyntheticInput": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_uint64, ct.c_int, ct.c_int]),
     "shafftAmdBatchPlan": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
                                       ct.c_int]),
     "shafftAmdBatchLayout": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
@@ -284,6 +285,19 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+UNIFORM, GAUSSIAN = 0, 1
+
+
+def synthetic_input(count: int, seed: int, prec: int, distribution: int = UNIFORM, L=None) -> np.ndarray:
+    """Benchmark input from std::mt19937_64(seed) (shafftAmdSyntheticInput):
+    uniform [-1, 1) or Box-Muller Gaussian re/im per element."""
+    L = L or _L()
+    out = np.empty(int(count), dtype=_np_dtype(prec))
+    check(L.shafftAmdSyntheticInput(out.ctypes.data, int(count), int(seed), distribution, prec),
+          "shafftAmdSyntheticInput", L)
+    return out
 
 
 def download_ptr(ptr: int, count: int, prec: int, L=None) -> np.ndarray:

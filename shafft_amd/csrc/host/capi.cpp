@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <random>
 #include <vector>
 
 #include "../../../include/shafft_amd.h"
@@ -322,6 +323,48 @@ int shafftAmd1DDescribe(shafft_amd_1d_plan* plan, char* buf, int buflen) {
   return guarded([&] {
     if (!plan) fail(ERR_NULLPTR, "null plan");
     return copy_desc(plan->be->program().describe(), buf, buflen);
+  });
+}
+
+// Synthetic inputs of the benchmark workloads: std::mt19937_64(seed), the
+// generator the reference's tests draw from (tests/integration/cpp/
+// test_correctness.cpp:382), with the value transform spelled out rather
+// than left to std::*_distribution (implementation-defined):
+//   distribution 0: per element re then im, u = (x >> 11) 2^-53, value 2u - 1;
+//   distribution 1: Box-Muller per element, u1 = ((x0 >> 11) + 1) 2^-53,
+//                   u2 = (x1 >> 11) 2^-53, r = sqrt(-2 ln u1),
+//                   value r (cos 2 pi u2 + i sin 2 pi u2).
+// Values are formed in double and rounded once for complex float.
+int shafftAmdSyntheticInput(void* host_out, int64_t count, uint64_t seed, int distribution,
+                            int precision) {
+  return guarded([&] {
+    if (!host_out && count > 0) fail(ERR_NULLPTR, "null output");
+    if (count < 0) fail(ERR_INVALID_DIM, "negative count");
+    if (distribution != 0 && distribution != 1) fail(ERR_INVALID_DIM, "unknown distribution");
+    if (precision != 0 && precision != 1) fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
+    std::mt19937_64 g(seed);
+    const double k53 = 1.0 / 9007199254740992.0, two_pi = 6.283185307179586476925286766559;
+    for (int64_t i = 0; i < count; ++i) {
+      double re, im;
+      if (distribution == 0) {
+        re = 2.0 * ((double)(g() >> 11) * k53) - 1.0;
+        im = 2.0 * ((double)(g() >> 11) * k53) - 1.0;
+      } else {
+        const double u1 = (double)((g() >> 11) + 1) * k53;
+        const double u2 = (double)(g() >> 11) * k53;
+        const double r = std::sqrt(-2.0 * std::log(u1));
+        re = r * std::cos(two_pi * u2);
+        im = r * std::sin(two_pi * u2);
+      }
+      if (precision) {
+        static_cast<double*>(host_out)[2 * i] = re;
+        static_cast<double*>(host_out)[2 * i + 1] = im;
+      } else {
+        static_cast<float*>(host_out)[2 * i] = (float)re;
+        static_cast<float*>(host_out)[2 * i + 1] = (float)im;
+      }
+    }
+    return 0;
   });
 }
 

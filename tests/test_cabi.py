@@ -110,3 +110,21 @@ def test_every_fortran_bound_symbol_is_exported():
     assert len(names) > 90
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_synthetic_input_matches_oracle_recipe(emu_lib):
+    """The library's benchmark input generator (std::mt19937_64) gives the
+    same values as the oracle's independent generator, both recipes and
+    precisions, bit for bit."""
+    import numpy as np
+
+    import oracle
+    from shafft_amd import api
+
+    for prec, dt in ((api.C2C, np.complex64), (api.Z2Z, np.complex128)):
+        u = api.synthetic_input(1000, 43, prec, api.UNIFORM, L=emu_lib)
+        assert np.array_equal(u, oracle.uniform_input((1000,), 43, dt))
+        g = api.synthetic_input(1000, 44, prec, api.GAUSSIAN, L=emu_lib)
+        assert np.array_equal(g, oracle.gaussian_input((1000,), 44, dt))
+    assert emu_lib.shafftAmdSyntheticInput(None, 4, 1, 0, 0) != 0
+    assert emu_lib.shafftAmdSyntheticInput(None, 0, 1, 0, 0) == 0

@@ -5,7 +5,7 @@ import numpy as np
 
 import oracle
 from shafft_amd import api
-from tests.batch_shapes import draw_shapes, smooth7
+from shafft_amd.workloads import draw_shapes, smooth7
 
 
 def _run(L, shapes, prec, inplace=False, use_graph=True, direction=0):

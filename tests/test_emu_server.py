@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from shafft_amd import api
-from tests.serve_stream import balanced_dims, draw_stream, factor7, smooth7_between
+from shafft_amd.workloads import balanced_dims, draw_stream, factor7, smooth7_between
 
 TOL = {0: 1e-5, 1: 1e-12}
 DT = {0: np.complex64, 1: np.complex128}

@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 from shafft_amd import api
-from tests.batch_shapes import draw_shapes
+from shafft_amd.workloads import draw_shapes
 
 pytestmark = pytest.mark.gpu
 TOL = {api.C2C: 1e-5, api.Z2Z: 1e-12}

@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 from shafft_amd import api
-from tests.serve_stream import draw_stream
+from shafft_amd.workloads import draw_stream
 
 pytestmark = pytest.mark.gpu
 TOL = {0: 1e-5, 1: 1e-12}

@@ -14,6 +14,7 @@ import pytest
 
 import oracle
 from oracle import oracle as orc
+from tests.closed_forms import chirp, chirp_spectrum
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -117,3 +118,59 @@ def test_error_metric_rejects_planted_mistakes():
         assert not oracle.check_rel_error(bad, np.fft.fftn(x), 1e-5)
     # reference scaling: tolerance grows with sqrt(ln N)
     assert math.isclose(math.sqrt(math.log(120)), 2.188, rel_tol=1e-3)
+
+
+def test_gaussian_input_recipe():
+    """Box-Muller on the pinned raw stream, re-derived in numpy, and the
+    moments of 2^20 draws (mean 0, variance 1 per component)."""
+    raw = orc.mt64_raw(8, 44)
+    u1 = ((raw[0::2] >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * 2.0 ** -53
+    u2 = (raw[1::2] >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    r = np.sqrt(-2.0 * np.log(u1))
+    x = oracle.gaussian_input((4,), 44)
+    assert np.allclose(x.real, r * np.cos(2 * np.pi * u2), rtol=1e-15, atol=0)
+    assert np.allclose(x.imag, r * np.sin(2 * np.pi * u2), rtol=1e-15, atol=0)
+    big = oracle.gaussian_input((1 << 20,), 44)
+    for comp in (big.real, big.imag):
+        assert abs(comp.mean()) < 5e-3 and abs(comp.var() - 1) < 5e-3
+    assert abs(np.mean(big.real * big.imag)) < 5e-3
+    assert np.array_equal(oracle.gaussian_input((3,), 44, np.complex64), x[:3].astype(np.complex64))
+
+
+@pytest.mark.parametrize("n", [8, 64, 1000, 4096])
+def test_chirp_closed_form(n):
+    """For even n the DFT of exp(i pi m^2/n) is sqrt(n) e^{i pi/4}
+    exp(-i pi k^2/n) (quadratic Gauss sum; the chirp is n-periodic for even
+    n).  The GPU tests use this closed form as the all-bins check of the
+    2^27-point FFT1D."""
+    x = chirp(n)
+    ref = chirp_spectrum(n)
+    assert np.max(np.abs(oracle.dftn(x, None, -1) - ref)) < 1e-12 * np.sqrt(n)
+    m = np.arange(n)
+    assert np.max(np.abs(x - np.exp(1j * np.pi * ((m * m) % (2 * n)) / n))) < 1e-14
+    # the chunked forms used on large n match the whole-array forms
+    assert np.array_equal(np.concatenate([chirp(n, s, min(n, s + 24) - s) for s in range(0, n, 24)]), x)
+    assert np.array_equal(np.concatenate([chirp_spectrum(n, s, min(n, s + 24) - s) for s in range(0, n, 24)]), ref)
+
+
+def test_separable_spectrum():
+    """A sum of separable terms u (x) v (x) w transforms term by term to
+    DFT(u) (x) DFT(v) (x) DFT(w): the all-bins check of the 1024^3 FFTND."""
+    rng = np.random.default_rng(11)
+    dims = (6, 10, 8)
+    terms = [[rng.standard_normal(d) + 1j * rng.standard_normal(d) for d in dims] for _ in range(2)]
+    x = sum(np.einsum("i,j,k->ijk", *t) for t in terms)
+    spec = [[oracle.dftn(v, None, -1) for v in t] for t in terms]
+    ref = sum(np.einsum("i,j,k->ijk", *s) for s in spec)
+    assert np.max(np.abs(oracle.dftn(x, None, -1) - ref)) < 1e-11
+
+
+@pytest.mark.parametrize("n", [4099, 9001])
+def test_pocketfft_prime_lengths_vs_direct(n):
+    """numpy's FFT is the full-array reference for the Bluestein lengths on
+    the GPU; at prime lengths it runs its own chirp-z path, checked here
+    against the direct DFT."""
+    x = oracle.gaussian_input((n,), 44)
+    ref = oracle.dftn(x, None, -1)
+    assert oracle.rel_l2(np.fft.fft(x), ref) < 1e-14
+    assert oracle.rel_l2(np.fft.ifft(x) * n, oracle.dftn(x, None, 1)) < 1e-14
