@@ -2,20 +2,31 @@
 """Benchmark of the slab C2C FFTND path on one MI355X.
 
 Workload (the metric's configuration): FFTND C2C complex128 512x512x512 at
-one rank.  One step = forward execute + backward execute + normalize through
-the reference-style C API (each call host-synchronous, as in SHAFFT).
-GFLOP/s counts 5 N log2 N per direction (2 directions per step).
+one rank, input std::mt19937_64 seed 43 uniform [-1,1).  One step = forward
+execute + backward execute + normalize through the reference-style C API
+(each call host-synchronous, as in SHAFFT).  GFLOP/s counts 5 N log2 N per
+direction (2 directions per step).
 
 Also reported, on the same JSON line:
   * roofline: the dominant kernel (the forward pass kernel), its average
     launch duration from HIP events recorded on the plan's stream around the
     forward executes of the timed region, and its algorithmic bytes per
     launch (one read + one write of the 2^27 complex128 grid = 4.295 GB);
-  * cpu_baseline: the oracle's FP64 FFT (radix-2, OpenMP) timed on this
-    host on one forward transform of the same grid.
+  * serving: the served half of the metric -- p50/p99 latency of the mixed
+    stream (Poisson arrivals at 2000 jobs/s, sizes 2^10..2^26, 70% complex64)
+    from resident plans, its saturated throughput against the plain
+    per-call path on the same jobs, and the same for the sub-stream of jobs
+    up to 2^20 elements;
+  * cpu_baseline: FFTW is not installed, so the labelled fallback
+    scipy.fft (pocketfft) on the host threads, forward, backward and step
+    timed separately plus a 1-thread forward, with the oracle's radix-2 FFT
+    as a secondary figure.
+
+Other workloads: --workload fft1d | batch | serve.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-                       [--dims 512,512,512] [--prec z2z|c2c] [--calibrate-copy]
+                       [--no-serving] [--dims 512,512,512] [--prec z2z|c2c]
+                       [--calibrate-copy]
 """
 import argparse
 import ctypes as ct
@@ -71,25 +82,82 @@ class Hip:
         assert self.L.hipDeviceSynchronize() == 0
 
 
-def cpu_baseline(x, dims, prec, budget_s=10.0):
+def _median_times(fn, reps, warm):
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def _host_threads():
+    """Host threads the CPU legs may use: the job's CPU share (OMP_NUM_THREADS
+    when set), not every core the machine reports."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    return int(v) if v.isdigit() and int(v) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_baseline(x, dims, prec, reps=10, warm=2):
+    """The reference's CPU path (its FFTW backend: guru plan over every axis,
+    out-of-place execute, OpenMP normalize; src/cpp/fft/fftw_method/
+    fftnd_method.cpp:34-276, common/normalize.cpp:6-32) is not runnable here:
+    FFTW is not installed on this host.  The labelled fallback is scipy.fft
+    (pocketfft) with workers = the host threads, on the full grid: forward,
+    backward and the step (forward + backward with the 1/N normalisation
+    folded into ifftn), median of `reps` after `warm` warm-ups; then a
+    1-thread forward on two transforms; and the oracle's own radix-2 FFT
+    ("port") on a few forward transforms as a secondary figure."""
+    n = int(np.prod(dims))
+    flop = 5 * n * math.log2(n)
+    a = x.reshape(dims)
+    ncores = _host_threads()
+    out = {"unit": "GFLOP/s", "cores": ncores}
+    try:
+        import scipy
+        import scipy.fft as sfft
+    except ImportError:
+        sfft = None
+    if sfft is not None:
+        box = {}
+        fwd = lambda: box.__setitem__("y", sfft.fftn(a, workers=ncores))  # noqa: E731
+        t_f, _ = _median_times(fwd, reps, warm)
+        y = box.pop("y")
+        bwd = lambda: sfft.ifftn(y, norm="forward", workers=ncores)  # noqa: E731
+        t_b, _ = _median_times(bwd, reps, warm)
+        step = lambda: sfft.ifftn(sfft.fftn(a, workers=ncores), workers=ncores)  # noqa: E731
+        t_s, _ = _median_times(step, max(3, reps // 2), 1)
+        del y
+        t_1, _ = _median_times(lambda: sfft.fftn(a, workers=1), 2, 0)
+        out.update({
+            "value": round(2 * flop / t_s / 1e9, 3),
+            "kind": "port",
+            "library": f"pocketfft (scipy {scipy.__version__}) fallback; FFTW unavailable",
+            "forward_ms": round(t_f * 1e3, 1),
+            "backward_ms": round(t_b * 1e3, 1),
+            "step_ms": round(t_s * 1e3, 1),
+            "forward_gflops": round(flop / t_f / 1e9, 3),
+            "backward_gflops": round(flop / t_b / 1e9, 3),
+            "forward_gflops_1thread": round(flop / t_1 / 1e9, 3),
+            "sample": f"full {'x'.join(map(str, dims))} grid in FP64; step = fftn + ifftn (1/N folded in), "
+                      f"median of {max(3, reps // 2)} after 1 warm-up; forward and backward: median of {reps} "
+                      f"after {warm} warm-ups, workers={ncores}; 1 thread: median of 2 forward transforms",
+        })
     import oracle  # the CPU-baseline leg is the only bench use of the oracle
 
-    a = x.reshape(dims).astype(np.complex128)
-    reps, t_total = 0, 0.0
-    while reps < 3 or (t_total < budget_s and reps < 50):
-        t0 = time.perf_counter()
-        oracle.fftn_pow2(a, None, -1)
-        t_total += time.perf_counter() - t0
-        reps += 1
-    n = a.size
-    return {
-        "value": round(reps * 5 * n * math.log2(n) / t_total / 1e9, 3),
-        "unit": "GFLOP/s",
-        "cores": oracle.threads(),
-        "kind": "port",
-        "sample": f"{reps} forward FFTND transforms of the full {'x'.join(map(str, dims))} grid in FP64 "
-                  f"({t_total:.1f} s), oracle radix-2 FFT with OpenMP over lines; FFTW is not installed",
-    }
+    a128 = a.astype(np.complex128, copy=False)
+    t_p, ts = _median_times(lambda: oracle.fftn_pow2(a128, None, -1), 3, 0)
+    port = {"forward_gflops": round(flop / t_p / 1e9, 3), "cores": oracle.threads(),
+            "sample": "median of 3 forward transforms of the full grid, the oracle's FP64 radix-2 FFT "
+                      "with OpenMP over lines"}
+    if sfft is None:
+        out.update({"value": port["forward_gflops"], "kind": "port",
+                    "sample": "scipy not importable; FFTW unavailable: " + port["sample"]})
+    else:
+        out["oracle_port"] = port
+    return out
 
 
 def profiled_traffic(dims, prec):
@@ -112,16 +180,13 @@ def bench_batch(args, hip):
     graph launch against the plain per-call path (one FFTND plan per shape,
     host-synchronous execute per transform, as the reference API runs)."""
     from shafft_amd import api
-    from tests.batch_shapes import draw_shapes
+    from shafft_amd.workloads import draw_shapes
 
     shapes = draw_shapes(args.count, 45)
     prec = api.C2C
     bp = api.BatchPlan(shapes, prec)
     tot = bp.total
-    rng = np.random.default_rng(45)
-    x = np.empty(tot, np.complex64)
-    x.real = rng.uniform(-1, 1, tot)
-    x.imag = rng.uniform(-1, 1, tot)
+    x = api.synthetic_input(tot, 45, prec, api.UNIFORM)  # std::mt19937_64 seed 45
     a, o, w = (api.DeviceBuffer(tot, prec) for _ in range(3))
     a.upload(x)
     stream = hip.stream()
@@ -238,9 +303,7 @@ def bench_fft1d(args, hip):
 
     n = args.n
     prec = api.Z2Z if args.prec == "z2z" else api.C2C
-    rng = np.random.default_rng(44)
-    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(
-        np.complex128 if prec == api.Z2Z else np.complex64)
+    x = api.synthetic_input(n, 44, prec, api.GAUSSIAN)  # std::mt19937_64 seed 44, Box-Muller
     p = api.OneDBackendPlan(n, prec)
     stream = hip.stream()
     p.set_stream(stream)
@@ -345,31 +408,68 @@ def _lat_stats(res):
             "max_ms": round(float(lat[-1]), 3), "span_s": span}
 
 
-def bench_serve(args, hip):
-    """Serving stream: Poisson arrivals of FFTND/FFT1D jobs, sizes
-    log-uniform in 2^10..2^26 elements (7-smooth), 70% complex64 / 30%
-    complex128, forward or backward.  Reports the p50/p99 latency at the
-    stated arrival rate (plans built on demand, then resident), the
-    throughput with every job queued at once, and the plain per-call path
-    (the reference call sequence: one plan per shape, host-synchronous
-    execute, one job at a time) on the same jobs."""
+def _serve_inputs(reqs):
     from shafft_amd import api
-    from tests.serve_stream import draw_stream
 
-    reqs = draw_stream(args.requests, seed=46, rate=args.rate)
     max_e = max(r.elems for r in reqs)
-    rng = np.random.default_rng(46)
     inp = {}
-    for prec, dt in ((api.C2C, np.complex64), (api.Z2Z, np.complex128)):
-        x = np.empty(max_e, dt)
-        x.real = rng.uniform(-1, 1, max_e)
-        x.imag = rng.uniform(-1, 1, max_e)
+    for prec in (api.C2C, api.Z2Z):
         inp[prec] = api.DeviceBuffer(max_e, prec)
-        inp[prec].upload(x)
-        del x
-    outs = [api.DeviceBuffer(max_e, api.Z2Z) for _ in range(args.outstanding)]
+        inp[prec].upload(api.synthetic_input(max_e, 46, prec, api.UNIFORM))  # std::mt19937_64 seed 46
+    return max_e, inp
+
+
+def _plain_run(reqs, hip, src, dst, with_planning=False):
+    """The plain per-call path on the same jobs: the reference call sequence
+    (src/cpp/shafft_internal.cpp:62-127), one plan per shape and one
+    host-synchronous execute per job, one job at a time.  Plans are made
+    beforehand (not timed) unless with_planning, where every job runs init,
+    plan, execute and destroy as for a shape not seen before."""
+    from shafft_amd import api
+
+    stream = hip.stream()
+
+    def make(r):
+        if r.kind == 1:
+            p = api.FFT1D()
+            p.init(r.n, r.n, 0, r.prec)
+        else:
+            p = api.FFTND()
+            p.init([1] * len(r.dims), r.dims, r.prec)
+        p.plan()
+        p.set_stream(stream)
+        return p
+
+    key = lambda r: (r.kind, r.n, tuple(r.dims or ()), r.prec)  # noqa: E731
+    plans = {}
+    if not with_planning:
+        for r in reqs:
+            if key(r) not in plans:
+                plans[key(r)] = make(r)
+    hip.sync()
+    t0 = time.perf_counter()
+    for r in reqs:
+        p = make(r) if with_planning else plans[key(r)]
+        p.set_buffers(src.ptr, dst.ptr)
+        p.execute(api.BACKWARD if r.direction else api.FORWARD)  # host-synchronous
+        if with_planning:
+            p.release()
+    hip.sync()
+    return time.perf_counter() - t0
+
+
+def serve_measure(reqs, hip, slots=8, arena_gb=24, outstanding=48, full=True):
+    """Serves `reqs` (each job reads a shared input and writes one of
+    `outstanding` output buffers): a cold paced run that builds the plans,
+    a paced run from resident plans (latency), every job queued at once
+    (throughput), and the plain per-call path on the same jobs.  full adds
+    the cold saturated run and the plain path with planning per call."""
+    from shafft_amd import api
+
+    max_e, inp = _serve_inputs(reqs)
+    outs = [api.DeviceBuffer(max_e, api.Z2Z) for _ in range(outstanding)]
     gflop = sum(r.gflop for r in reqs)
-    srv = api.Server(slots=args.slots, arena_bytes=args.arena_gb << 30, planners=2)
+    srv = api.Server(slots=slots, arena_bytes=arena_gb << 30, planners=2)
     cold = _lat_stats(_serve_run(srv, reqs, inp, outs, paced=True))
     st_cold = srv.stats()
     srv.reset_stats()
@@ -380,98 +480,100 @@ def bench_serve(args, hip):
     st = srv.stats()
     srv.close()
     gpu_busy_s = sum(r.gpu_ns for r in sat_res) * 1e-9
-    # every job queued at once on a fresh server: plans built while serving
-    srv = api.Server(slots=args.slots, arena_bytes=args.arena_gb << 30, planners=2)
-    cold_sat = _lat_stats(_serve_run(srv, reqs, inp, outs, paced=False))
-    srv.close()
+    # the plain path may overwrite its source (execute leaves the result in
+    # whichever buffer it ends in); the served runs above are done with it
+    plain = _plain_run(reqs, hip, inp[api.Z2Z], outs[0])
+    out = {
+        "jobs": len(reqs),
+        "total_gflop": round(gflop, 2),
+        "p50_ms": warm["p50_ms"],
+        "p99_ms": warm["p99_ms"],
+        "mean_ms": warm["mean_ms"],
+        "max_ms": warm["max_ms"],
+        "paced_gflops": round(gflop / warm["span_s"], 2),
+        "cold_p50_ms": cold["p50_ms"],
+        "cold_p99_ms": cold["p99_ms"],
+        "cold_plan_builds": st_cold["plan_builds"],
+        "saturated_gflops": round(gflop / sat["span_s"], 2),
+        "saturated_req_per_s": round(len(reqs) / sat["span_s"], 1),
+        "saturated_p50_ms": sat["p50_ms"],
+        "saturated_p99_ms": sat["p99_ms"],
+        "gpu_busy_frac": round(gpu_busy_s / sat["span_s"], 3),
+        "plain_per_call_s": round(plain, 4),
+        "plain_per_call_gflops": round(gflop / plain, 2),
+        "plain_per_call_req_per_s": round(len(reqs) / plain, 1),
+        "speedup_vs_plain": round(plain / sat["span_s"], 3),
+        "plans_resident": st["plans"],
+        "table_mb": round(st["table_bytes"] / 2**20, 2),
+        "arena_peak_gb": round(st["arena_peak"] / 2**30, 3),
+    }
+    if full:
+        srv = api.Server(slots=slots, arena_bytes=arena_gb << 30, planners=2)
+        cold_sat = _lat_stats(_serve_run(srv, reqs, inp, outs, paced=False))
+        srv.close()
+        plain_planned = _plain_run(reqs, hip, inp[api.Z2Z], outs[0], with_planning=True)
+        out.update({"cold_saturated_s": round(cold_sat["span_s"], 4),
+                    "plain_with_planning_s": round(plain_planned, 4),
+                    "cold_speedup_vs_plain_with_planning": round(plain_planned / cold_sat["span_s"], 3)})
+    return out
 
-    # plain per-call path on the same jobs: plans made beforehand (not timed)
-    stream = hip.stream()
-    plans = {}
-    for r in reqs:
-        key = (r.kind, r.n, tuple(r.dims or ()), r.prec)
-        if key in plans:
-            continue
-        if r.kind == 1:
-            p = api.FFT1D()
-            p.init(r.n, r.n, 0, r.prec)
-        else:
-            p = api.FFTND()
-            p.init([1] * len(r.dims), r.dims, r.prec)
-        p.plan()
-        p.set_stream(stream)
-        plans[key] = p
-    scratch = api.DeviceBuffer(max_e, api.Z2Z)
-    hip.sync()
-    t0 = time.perf_counter()
-    for r in reqs:
-        p = plans[(r.kind, r.n, tuple(r.dims or ()), r.prec)]
-        p.set_buffers(scratch.ptr, outs[0].ptr)
-        p.execute(api.BACKWARD if r.direction else api.FORWARD)  # host-synchronous
-    hip.sync()
-    plain = time.perf_counter() - t0
-    del plans
-    # plain per-call including the plan: init, plan, execute, destroy per job
-    # (the reference call sequence for a shape not seen before)
-    hip.sync()
-    t0 = time.perf_counter()
-    for r in reqs:
-        if r.kind == 1:
-            p = api.FFT1D()
-            p.init(r.n, r.n, 0, r.prec)
-        else:
-            p = api.FFTND()
-            p.init([1] * len(r.dims), r.dims, r.prec)
-        p.plan()
-        p.set_stream(stream)
-        p.set_buffers(scratch.ptr, outs[0].ptr)
-        p.execute(api.BACKWARD if r.direction else api.FORWARD)
-        p.release()
-    hip.sync()
-    plain_planned = time.perf_counter() - t0
 
+STREAM_DESC = ("Poisson arrivals at {rate:g}/s, sizes log-uniform 2^{lo}..2^{hi} (7-smooth), 70% complex64 / "
+               "30% complex128, fwd/bwd 50/50, half FFTND 2-D/3-D, half FFT1D")
+
+
+def serving_summary(hip, requests=2000, rate=2000.0):
+    """The serving half of the metric: the mixed stream at its configured
+    range (2^10..2^26) and the sub-stream of jobs up to 2^20 elements, where
+    per-call overheads rather than HBM bandwidth bound the plain path."""
+    from shafft_amd.workloads import draw_stream
+
+    full = serve_measure(draw_stream(requests, seed=46, rate=rate), hip, full=False)
+    small = serve_measure(draw_stream(requests, seed=46, rate=rate, hi=20), hip, full=False)
+    return {
+        "workload": f"{requests} jobs, " + STREAM_DESC.format(rate=rate, lo=10, hi=26),
+        **{k: full[k] for k in ("p50_ms", "p99_ms", "cold_p50_ms", "cold_p99_ms", "saturated_gflops",
+                                "saturated_req_per_s", "plain_per_call_gflops", "speedup_vs_plain",
+                                "gpu_busy_frac")},
+        "target_speedup_vs_plain": 3.0,
+        "substream_le_2p20": {
+            "workload": f"{requests} jobs, " + STREAM_DESC.format(rate=rate, lo=10, hi=20),
+            **{k: small[k] for k in ("p50_ms", "p99_ms", "saturated_req_per_s", "saturated_gflops",
+                                     "plain_per_call_req_per_s", "plain_per_call_gflops", "speedup_vs_plain")},
+        },
+    }
+
+
+def bench_serve(args, hip):
+    """Serving stream: Poisson arrivals of FFTND/FFT1D jobs, sizes
+    log-uniform in 2^lo..2^hi elements (7-smooth), 70% complex64 / 30%
+    complex128, forward or backward.  Reports the p50/p99 latency at the
+    stated arrival rate (plans built on demand, then resident), the
+    throughput with every job queued at once, and the plain per-call path
+    (the reference call sequence: one plan per shape, host-synchronous
+    execute, one job at a time) on the same jobs."""
+    from shafft_amd.workloads import draw_stream
+
+    reqs = draw_stream(args.requests, seed=46, rate=args.rate, hi=args.serve_hi)
+    m = serve_measure(reqs, hip, slots=args.slots, arena_gb=args.arena_gb, outstanding=args.outstanding)
     out = {
         "metric": "served mixed-size stream throughput (GFLOP/s, 5 N log2 N per job); p50/p99 latency",
-        "value": round(gflop / sat["span_s"], 2),
+        "value": m["saturated_gflops"],
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": 1,
         "warmup": 0,
-        "ms_per_step": round(sat["span_s"] * 1e3, 3),
+        "ms_per_step": round(m["total_gflop"] / m["saturated_gflops"] * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{len(reqs)} jobs, Poisson arrivals at {args.rate:g}/s, sizes log-uniform 2^10..2^26 "
-                        f"(7-smooth), 70% complex64 / 30% complex128, fwd/bwd 50/50, half FFTND 2-D/3-D, "
-                        f"half FFT1D",
+            "workload": f"{len(reqs)} jobs, " + STREAM_DESC.format(rate=args.rate, lo=10, hi=args.serve_hi),
             "slots": args.slots,
             "outstanding_buffers": args.outstanding,
-            "total_gflop": round(gflop, 2),
-            "p50_ms": warm["p50_ms"],
-            "p99_ms": warm["p99_ms"],
-            "mean_ms": warm["mean_ms"],
-            "max_ms": warm["max_ms"],
-            "paced_gflops": round(gflop / warm["span_s"], 2),
-            "cold_p50_ms": cold["p50_ms"],
-            "cold_p99_ms": cold["p99_ms"],
-            "cold_plan_builds": st_cold["plan_builds"],
-            "saturated_req_per_s": round(len(reqs) / sat["span_s"], 1),
-            "saturated_p50_ms": sat["p50_ms"],
-            "saturated_p99_ms": sat["p99_ms"],
-            "gpu_busy_frac": round(gpu_busy_s / sat["span_s"], 3),
-            "plain_per_call_s": round(plain, 4),
-            "plain_per_call_gflops": round(gflop / plain, 2),
-            "plain_per_call_req_per_s": round(len(reqs) / plain, 1),
-            "speedup_vs_plain": round(plain / sat["span_s"], 3),
-            "cold_saturated_s": round(cold_sat["span_s"], 4),
-            "plain_with_planning_s": round(plain_planned, 4),
-            "cold_speedup_vs_plain_with_planning": round(plain_planned / cold_sat["span_s"], 3),
-            "plans_resident": st["plans"],
-            "table_mb": round(st["table_bytes"] / 2**20, 2),
-            "arena_peak_gb": round(st["arena_peak"] / 2**30, 3),
+            **m,
         },
     }
     print(json.dumps(out), flush=True)
@@ -492,6 +594,9 @@ def main():
     ap.add_argument("--slots", type=int, default=8, help="server stream slots")
     ap.add_argument("--outstanding", type=int, default=48, help="serve output buffers (max job size each)")
     ap.add_argument("--arena-gb", type=int, default=24, help="server scratch arena (GiB)")
+    ap.add_argument("--serve-hi", type=int, default=26, help="serve: largest job is 2^serve_hi elements")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="fftnd: skip the serving-stream summary on the JSON line")
     ap.add_argument("--count", type=int, default=4096, help="transforms in the batch workload")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batch-modes", default="fp32,bf16_split,fp16_split",
@@ -525,10 +630,7 @@ def main():
     hip = Hip()
     stream = hip.stream()
 
-    rng = np.random.default_rng(43)
-    x = np.empty(n, np.complex128 if prec == api.Z2Z else np.complex64)
-    x.real = rng.uniform(-1, 1, n)
-    x.imag = rng.uniform(-1, 1, n)
+    x = api.synthetic_input(n, 43, prec, api.UNIFORM)  # std::mt19937_64 seed 43, uniform [-1,1)
 
     plan = api.FFTND()
     plan.init([1] * len(dims), dims, prec)
@@ -612,6 +714,9 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 4),
         },
     }
+    del a, b, plan
+    if not args.no_serving:
+        out["serving"] = serving_summary(hip)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, dims, prec)
     print(json.dumps(out), flush=True)
