@@ -181,7 +181,10 @@ void BatchPlan::build() {
   up(d_dep_, dep.data(), np * sizeof(int32_t));
   up(d_items_, items_.data(), items_.size() * sizeof(sa_batch_item_t));
   if (d_done_.bytes < std::max<size_t>(np, 1) * 4) d_done_ = DevBuf(std::max<size_t>(np, 1) * 4);
-  if (!d_head_.p) d_head_ = DevBuf(256);  // [0] queue head, [1] wait-timeout flag
+  if (!d_head_.p) {
+    d_head_ = DevBuf(256);  // [0] queue head, [1] wait-timeout flag (sticky)
+    hip_check(hipMemset(d_head_.p, 0, 256), "hipMemset(batch flag)");
+  }
   built_ = true;
 }
 
@@ -269,13 +272,23 @@ void BatchPlan::execute(int dir, bool use_graph) {
   hip_check(hipGraphLaunch(graph_[d], stream_), "hipGraphLaunch");
 }
 
+// The wait-timeout flag is set by any launch since the last check (the
+// kernels never clear it); reading it here clears it, so one timed-out
+// launch fails exactly one synchronize.
+void BatchPlan::check_timeout_flag() {
+  if (!d_head_.p) return;
+  uint32_t* flag = static_cast<uint32_t*>(d_head_.p) + 1;
+  uint32_t err = 0;
+  hip_check(hipMemcpy(&err, flag, 4, hipMemcpyDeviceToHost), "hipMemcpy(batch flag)");
+  if (err) {
+    hip_check(hipMemset(flag, 0, 4), "hipMemset(batch flag)");
+    fail(ERR_BACKEND, "batch launch: a dependency wait timed out");
+  }
+}
+
 void BatchPlan::synchronize() {
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  if (!d_head_.p) return;
-  uint32_t err = 0;
-  hip_check(hipMemcpy(&err, static_cast<uint32_t*>(d_head_.p) + 1, 4, hipMemcpyDeviceToHost),
-            "hipMemcpy(batch flag)");
-  if (err) fail(ERR_BACKEND, "batch launch: a dependency wait timed out");
+  check_timeout_flag();
 }
 
 int64_t BatchPlan::total_bytes() const { return prog_.total_bytes(); }
@@ -302,6 +315,7 @@ std::string BatchPlan::autotune() {
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
     ms /= 3;
+    check_timeout_flag();
     os << "block=" << c.block << " oversub=" << c.oversub << " mode=" << (c.levels ? "levels" : "deps")
        << " grid=" << grid_ << " : " << ms << " ms\n";
     if (ms < best) {

@@ -40,7 +40,8 @@ class BatchPlan {
   hipStream_t stream() const { return stream_; }
   // dir -1 forward / +1 backward; graph replay unless use_graph is false.
   void execute(int dir, bool use_graph);
-  // Wait for the stream; fails if the last launch reported a timed-out wait.
+  // Wait for the stream; fails if any launch since the last check reported
+  // a timed-out dependency wait (the flag is sticky until read).
   void synchronize();
   // Radix-8/16 stages: 0 FP32 (default), 1 bfloat16 hi/lo split on the
   // matrix cores, 2 float16 hi/lo split (complex float batches only).
@@ -80,6 +81,7 @@ class BatchPlan {
   void drop_graphs();
   void launch(int dir, hipStream_t s);
   void copy_identities(hipStream_t s);  // transforms with no pass
+  void check_timeout_flag();            // read and clear the device flag
 };
 
 }  // namespace sa

@@ -67,14 +67,17 @@ int shafftGetCommunicatorf03(void* planPtr, shafft_fint_t* outFint) {
   return rc;
 }
 
-#define SA_GET(T, CFG, NAME, EXPR) \
-  T shafftConfig##NAME(const CFG* cfg) { return cfg ? (T)(EXPR) : (T)0; }
+// Each accessor returns its field, or the reference's value for a null
+// config (src/f03/shafftf03.cpp: 0, and -1 for activeRank).
+#define SA_GET_OR(T, CFG, NAME, EXPR, NONE) \
+  T shafftConfig##NAME(const CFG* cfg) { return cfg ? (T)(EXPR) : (T)(NONE); }
+#define SA_GET(T, CFG, NAME, EXPR) SA_GET_OR(T, CFG, NAME, EXPR, 0)
 
 SA_GET(int, shafft_nd_config_t, NDGetNdim, cfg->ndim)
 SA_GET(int, shafft_nd_config_t, NDGetPrecision, cfg->precision)
 SA_GET(size_t, shafft_nd_config_t, NDGetAllocElements, cfg->allocElements)
 SA_GET(int, shafft_nd_config_t, NDGetIsActive, cfg->isActive)
-SA_GET(int, shafft_nd_config_t, NDGetActiveRank, cfg->activeRank)
+SA_GET_OR(int, shafft_nd_config_t, NDGetActiveRank, cfg->activeRank, -1)
 SA_GET(int, shafft_nd_config_t, NDGetActiveSize, cfg->activeSize)
 SA_GET(int, shafft_nd_config_t, NDGetNda, cfg->nda)
 SA_GET(int, shafft_nd_config_t, NDGetCommSize, cfg->commSize)
@@ -94,7 +97,7 @@ SA_GET(size_t, shafft_1d_config_t, 1DGetGlobalSize, cfg->globalSize)
 SA_GET(int, shafft_1d_config_t, 1DGetPrecision, cfg->precision)
 SA_GET(size_t, shafft_1d_config_t, 1DGetAllocElements, cfg->allocElements)
 SA_GET(int, shafft_1d_config_t, 1DGetIsActive, cfg->isActive)
-SA_GET(int, shafft_1d_config_t, 1DGetActiveRank, cfg->activeRank)
+SA_GET_OR(int, shafft_1d_config_t, 1DGetActiveRank, cfg->activeRank, -1)
 SA_GET(int, shafft_1d_config_t, 1DGetActiveSize, cfg->activeSize)
 SA_GET(int, shafft_1d_config_t, 1DGetStatus, cfg->status)
 SA_GET(int, shafft_1d_config_t, 1DGetFlags, cfg->flags)

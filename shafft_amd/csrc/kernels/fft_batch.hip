@@ -20,13 +20,13 @@
 
 namespace sa {
 
-__global__ void batch_clear(uint32_t* done, uint32_t n, uint32_t* head, uint32_t* err) {
+// Clears the per-pass completion counters and the work-queue head.  The
+// wait-timeout flag is not cleared here: it stays set across launches (and
+// graph replays) until the host has read it.
+__global__ void batch_clear(uint32_t* done, uint32_t n, uint32_t* head) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) done[i] = 0u;
-  if (i == 0) {
-    *head = 0u;
-    *err = 0u;
-  }
+  if (i == 0) *head = 0u;
 }
 
 template <typename R, int MX>
@@ -88,7 +88,7 @@ extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int bloc
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const uint32_t n = b->npasses;
   hipLaunchKernelGGL(sa::batch_clear, dim3((n + 255) / 256 ? (n + 255) / 256 : 1), dim3(256), 0, st,
-                     b->done, n, b->head, b->err);
+                     b->done, n, b->head);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return (int)e;
   // 160 KiB minus room for the kernel's static LDS (the queue item slot);

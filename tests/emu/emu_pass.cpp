@@ -330,7 +330,6 @@ extern "C" int sa_launch_batch(const sa_batch_t* bp, int prec, int grid, int blo
     };
     if (grid < 1 || block < 1 || block > 512 || lds > 159 * 1024) die("launch shape");
     if (!b.err || b.wait_ticks == 0) die("missing wait bound");
-    *b.err = 0;
     bool first_level = true;
     for (uint32_t it = 0; it < b.nitems; ++it)
       if (b.pass_dep[b.items[it].pass] >= 0) first_level = false;
@@ -382,6 +381,10 @@ hipError_t hipFree(void* p) {
   std::lock_guard<std::mutex> lk(allocs_mu());
   allocs().erase((uintptr_t)p);
   std::free(p);
+  return hipSuccess;
+}
+hipError_t hipMemset(void* d, int v, size_t n) {
+  std::memset(d, v, n);
   return hipSuccess;
 }
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {

@@ -98,6 +98,8 @@ def test_fortran_config_accessors():
     L.shafftConfig1DRelease(c1)
     L.shafftConfig1DFreef03(c1)
     assert L.shafftConfigNDGetNdim(None) == 0  # null config reads as 0
+    # ... except activeRank, -1 as in the reference's src/f03/shafftf03.cpp
+    assert L.shafftConfigNDGetActiveRank(None) == -1 and L.shafftConfig1DGetActiveRank(None) == -1
 
 
 def test_every_fortran_bound_symbol_is_exported():
