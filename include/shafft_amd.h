@@ -78,6 +78,20 @@ int shafftAmd1DPlan(shafft_amd_1d_plan** plan, size_t globalN, int precision, vo
  * preserved unless in == out.  Asynchronous on the plan's stream. */
 int shafftAmd1DExecute(shafft_amd_1d_plan* plan, void* in, void* out, int direction);
 
+/* Allocation-free variant of shafftAmd1DExecute, for callers that cannot
+ * allocate device memory while they run (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No reference
+ * counterpart.  Same transform, arguments and kernels as shafftAmd1DExecute,
+ * but the plan's intermediates (the Bluestein convolution, the in-place and
+ * input-preserving four-step temporaries) go to one library-owned device
+ * array of 64 MiB per GPU (held until the process ends) instead of plan
+ * scratch, and the call allocates nothing.  A plan needing more fails with
+ * SHAFFT_ERR_ALLOC (e.g. 1,000,003 points fit in both precisions; 2^27
+ * points do not).  Every call on a GPU shares that array: calls that
+ * overlap on one GPU race.  Asynchronous on the plan's stream. */
+int shafftAmd1DExecuteNoAlloc(shafft_amd_1d_plan* plan, const void* in, void* out,
+                              int direction);
+
 /* Replace shafft::fft1dSetStream / fft1dSynchronize / fft1dDestroy. */
 int shafftAmd1DSetStream(shafft_amd_1d_plan* plan, void* stream);
 int shafftAmd1DSynchronize(shafft_amd_1d_plan* plan);

@@ -47,6 +47,7 @@ _SIGS = {
     "shafftAmd1DQueryLayout": (ct.c_int, [ct.c_size_t] + [ct.POINTER(ct.c_size_t)] * 5),
     "shafftAmd1DPlan": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_size_t, ct.c_int, ct.c_void_p, ct.c_void_p]),
     "shafftAmd1DExecute": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int]),
+    "shafftAmd1DExecuteNoAlloc": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int]),
     "shafftAmd1DSetStream": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "shafftAmd1DSynchronize": (ct.c_int, [ct.c_void_p]),
     "shafftAmd1DDestroy": (ct.c_int, [ct.c_void_p]),
@@ -453,6 +454,12 @@ class OneDBackendPlan:
     def execute(self, src: int, dst: int, direction: int):
         check(self.L.shafftAmd1DExecute(self.h, ct.c_void_p(src), ct.c_void_p(dst), direction),
               "shafftAmd1DExecute", self.L)
+
+    def execute_no_alloc(self, src: int, dst: int, direction: int):
+        """shafftAmd1DExecuteNoAlloc: the same transform with the
+        intermediates in the library's fixed device array (no allocation)."""
+        check(self.L.shafftAmd1DExecuteNoAlloc(self.h, ct.c_void_p(src), ct.c_void_p(dst), direction),
+              "shafftAmd1DExecuteNoAlloc", self.L)
 
     def synchronize(self):
         check(self.L.shafftAmd1DSynchronize(self.h), "shafftAmd1DSynchronize", self.L)

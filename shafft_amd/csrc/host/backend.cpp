@@ -219,6 +219,30 @@ void OneDBackend::execute(void* in, void* out, int dir) {
   prog_.run(m, slots, dir, stream_);
 }
 
+void OneDBackend::execute_in_scratch(const void* in, void* out, int dir, void* scratch,
+                                     size_t scratch_bytes) const {
+  if (!in || !out) fail(ERR_NO_BUFFER, "buffer not set");
+  const bool inplace = in == out;
+  const PassProgram::SlotMap& m = inplace ? map_ip_ : map_pr_;
+  const size_t esz = prec_ ? 16 : 8;
+  const size_t b0 = ((size_t)m.scratch[0] * esz + 255) & ~size_t(255);
+  const size_t need = b0 + (size_t)m.scratch[1] * esz;
+  if (need > scratch_bytes)
+    fail(ERR_ALLOC, "FFT1D of length " + std::to_string(n_) + " needs " + std::to_string(need) +
+                        " bytes of scratch; the allocation-free execute has " +
+                        std::to_string(scratch_bytes));
+  if (prog_.npasses() == 0) {
+    if (!inplace)
+      hip_check(hipMemcpyAsync(out, in, (size_t)n_ * esz, hipMemcpyDeviceToDevice, stream_),
+                "hipMemcpyAsync");
+    return;
+  }
+  char* base = static_cast<char*>(scratch);
+  void* slots[SLOT_COUNT] = {const_cast<void*>(in), out, m.scratch[0] ? base : nullptr,
+                             m.scratch[1] ? base + b0 : nullptr};
+  prog_.run(m, slots, dir, stream_);
+}
+
 void OneDBackend::execute_preserve(const void* in, void* out, void* s1, void* s2, int dir,
                                    hipStream_t stream) const {
   run_preserve(prog_, map_pr_, n_, prec_, in, out, s1, s2, dir, stream);

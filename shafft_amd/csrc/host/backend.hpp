@@ -58,6 +58,11 @@ class OneDBackend {
   void execute(void* in, void* out, int dir);
   void execute_preserve(const void* in, void* out, void* s1, void* s2, int dir,
                         hipStream_t stream) const;
+  // execute() with the intermediates in the caller's `scratch` of
+  // `scratch_bytes` (no allocation); fails with ERR_ALLOC if the plan needs
+  // more.  Asynchronous on the plan's stream.
+  void execute_in_scratch(const void* in, void* out, int dir, void* scratch,
+                          size_t scratch_bytes) const;
   int64_t preserve_scratch_elems(int i) const { return map_pr_.scratch[i]; }
   int64_t table_bytes() const { return prog_.table_bytes(); }
   void set_stream(hipStream_t s) { stream_ = s; }

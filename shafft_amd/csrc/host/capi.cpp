@@ -186,6 +186,18 @@ int shafftAmd1DExecute(shafft_amd_1d_plan* plan, void* in, void* out, int direct
   });
 }
 
+int shafftAmd1DExecuteNoAlloc(shafft_amd_1d_plan* plan, const void* in, void* out,
+                              int direction) {
+  return guarded([&] {
+    if (!plan) fail(ERR_NULLPTR, "null plan");
+    void* scratch = nullptr;
+    size_t bytes = 0;
+    hip_check((hipError_t)sa_noalloc_scratch(&scratch, &bytes), "allocation-free scratch");
+    plan->be->execute_in_scratch(in, out, direction == 0 ? -1 : +1, scratch, bytes);
+    return 0;
+  });
+}
+
 int shafftAmd1DSetStream(shafft_amd_1d_plan* plan, void* stream) {
   return guarded([&] {
     if (!plan) fail(ERR_NULLPTR, "null plan");

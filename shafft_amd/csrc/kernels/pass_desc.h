@@ -152,6 +152,10 @@ int sa_launch_scale(void* data, double f, uint64_t count, int prec, void* stream
 /* Largest dynamic LDS a pass may use (bytes). */
 int sa_max_lds_bytes(void);
 
+/* Device address and size of the allocation-free execute's scratch array on
+ * the current device (resolved once per device, then cached). */
+int sa_noalloc_scratch(void** ptr, size_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -234,6 +234,21 @@ void run_pass(const sa_pass_t& p, int sign) {
 
 extern "C" int sa_max_lds_bytes(void) { return 160 * 1024; }
 
+// The allocation-free execute's device array, as one registered host block
+// of the same size (so every access is bounds-checked against it).
+extern "C" int sa_noalloc_scratch(void** ptr, size_t* bytes) {
+  static const size_t kBytes = size_t(64) << 20;
+  static void* block = [] {
+    void* p = std::malloc(kBytes);
+    std::lock_guard<std::mutex> lk(allocs_mu());
+    allocs()[(uintptr_t)p] = kBytes;
+    return p;
+  }();
+  *ptr = block;
+  *bytes = kBytes;
+  return 0;
+}
+
 extern "C" int sa_launch_pass(const sa_pass_t* p, int prec, int sign, void*) {
   if (prec == 0)
     run_pass<float>(*p, sign);
