@@ -20,6 +20,19 @@
 
 namespace sa {
 
+// Tile of this workgroup.  Workgroups are dealt to the 8 XCDs round robin
+// (workgroup b on XCD b % 8).  With xcd_order set, XCD x takes a contiguous
+// range of tiles instead (workgroup x + 8 i -> tile start_x + i, a bijection
+// on [0, grid)), so tiles next to each other run at the same time on one
+// XCD and the partial cache lines at the edges of their row segments meet
+// in one L2 instead of being fetched and written back from two.
+__device__ __forceinline__ uint32_t tile_of_block(const sa_pass_t& a) {
+  const uint32_t b = blockIdx.x;
+  if (!a.xcd_order) return b;
+  const uint32_t q = gridDim.x >> 3, r = gridDim.x & 7u, x = b & 7u;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // Register budget and occupancy.  With the stage loop's LDS addresses
 // computed inside each stage (stage_lane), the LDS-mode kernels need ~94
 // (complex float, radices up to 21) / ~151 (complex double, up to 28) VGPRs,
@@ -33,13 +46,13 @@ namespace sa {
 template <int SIGN, int MODE, bool TWL, int E = EMAX>
 __global__ __launch_bounds__(512) void fft_lds_pass_f(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<float, SIGN, MODE, 0, TWL, E>(a, blockIdx.x, reinterpret_cast<cpx<float>*>(lds_raw));
+  pass_tile<float, SIGN, MODE, 0, TWL, E>(a, tile_of_block(a), reinterpret_cast<cpx<float>*>(lds_raw));
 }
 
 template <int SIGN, int MODE, bool TWL, int E = EMAX>
 __global__ __launch_bounds__(512) void fft_lds_pass_d(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<double, SIGN, MODE, 0, TWL, E>(a, blockIdx.x, reinterpret_cast<cpx<double>*>(lds_raw));
+  pass_tile<double, SIGN, MODE, 0, TWL, E>(a, tile_of_block(a), reinterpret_cast<cpx<double>*>(lds_raw));
 }
 
 template <typename R, int SIGN, int MODE, bool TWL, int E = EMAX>
