@@ -105,9 +105,11 @@ static hipError_t launch_tw(const sa_pass_t* p, hipStream_t st) {
 template <typename R>
 static hipError_t launch_prec(const sa_pass_t* p, int sign, hipStream_t st) {
   if (p->blu_h) {
+    const bool twl = p->tw_lds > 0;
     if constexpr (sizeof(R) == 4)  // 8 elements per thread: complex float only
-      if (p->emax == 8) return launch_t<R, -1, MODE_BLUE, false, 8>(p, st);
-    return launch_t<R, -1, MODE_BLUE, false>(p, st);
+      if (p->emax == 8)
+        return twl ? launch_t<R, -1, MODE_BLUE, true, 8>(p, st) : launch_t<R, -1, MODE_BLUE, false, 8>(p, st);
+    return twl ? launch_t<R, -1, MODE_BLUE, true>(p, st) : launch_t<R, -1, MODE_BLUE, false>(p, st);
   }
   const bool regs = p->reg_first && p->reg_last;
   if (sign < 0) return regs ? launch_tw<R, -1, MODE_REGS>(p, st) : launch_tw<R, -1, MODE_LDS>(p, st);
@@ -123,8 +125,9 @@ template <typename R>
 static const void* pass_kernel(const sa_pass_t* p, bool twl) {
   if (p->blu_h) {
     if constexpr (sizeof(R) == 4)
-      if (p->emax == 8) return PassKernel<R, -1, MODE_BLUE, false, 8>::fn();
-    return PassKernel<R, -1, MODE_BLUE, false>::fn();
+      if (p->emax == 8)
+        return twl ? PassKernel<R, -1, MODE_BLUE, true, 8>::fn() : PassKernel<R, -1, MODE_BLUE, false, 8>::fn();
+    return twl ? PassKernel<R, -1, MODE_BLUE, true>::fn() : PassKernel<R, -1, MODE_BLUE, false>::fn();
   }
   if (p->reg_first && p->reg_last)
     return twl ? PassKernel<R, -1, MODE_REGS, true>::fn() : PassKernel<R, -1, MODE_REGS, false>::fn();

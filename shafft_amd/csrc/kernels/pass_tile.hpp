@@ -276,7 +276,10 @@ __device__ __forceinline__ void last_dispatch(const sa_pass_t& a, cpx<R> (&v)[EM
     case 4: stage_last_regs<R, 4, SIGN>(v, line, a.n, tws, tl, a.T, act); break;
     case 8: stage_last_regs<R, 8, SIGN>(v, line, a.n, tws, tl, a.T, act); break;
     case 16: stage_last_regs<R, 16, SIGN>(v, line, a.n, tws, tl, a.T, act); break;
-    default: break;
+    default:  // not reached: register stages need a radix dividing 16 at both ends
+#pragma unroll
+      for (int i = 0; i < EMAX; ++i) v[i] = {R(0), R(0)};
+      break;
   }
 }
 
@@ -322,9 +325,19 @@ __device__ __forceinline__ uint32_t slot_elem(const sa_pass_t& a, uint32_t t, bo
   return fdiv_d(t, a.Cdiv);
 }
 
+// Elements per group of the hooked loads and stores: the group's data and
+// table values are held in registers at once.
+template <typename R, int E>
+__device__ constexpr int hook_group() {
+  return E < 8 ? E : 8;
+}
+
 // Loads elements k = k0 + i*T (i < E, k < n) of one line into v, with the
-// load hooks applied (zero outside the hook's limit).
-template <typename R, int E = EMAX>
+// load hooks applied (zero outside the hook's limit).  FULL: n == E*T and
+// k0 < T, so every k is in the line and no element load is predicated (a
+// predicated load into the register array that the first stage then reads
+// makes the compiler park elements in scratch and wait on each load).
+template <typename R, int E = EMAX, bool FULL = false>
 __device__ __forceinline__ void load_line(const sa_pass_t& a, const LineOff& o, uint32_t k0,
                                           cpx<R> (&v)[E]) {
   const int n = a.n;
@@ -336,29 +349,41 @@ __device__ __forceinline__ void load_line(const sa_pass_t& a, const LineOff& o, 
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) v[i] = gload(src + (int64_t)k * a.iks, nt);
+      if (FULL || k < (uint32_t)n) v[i] = gload(src + (int64_t)k * a.iks, nt);
     }
   } else {
+    // Hooked elements in groups of HG: every data and table load of a group
+    // is issued before any is used.  (A table multiply inside each
+    // element's own conditional waits for that element's two loads, so the
+    // loads of a thread ran one after another at full memory latency.)
     const cpx<R>* __restrict__ pre = reinterpret_cast<const cpx<R>*>(a.pre);
     const int64_t base = o.ld_base;
+    constexpr int HG = hook_group<R, E>();
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) {
+    for (int g = 0; g < E; g += HG) {
+      cpx<R> x[HG], w[HG];
+#pragma unroll
+      for (int j = 0; j < HG; ++j) {
+        const uint32_t k = k0 + (g + j) * T;
         const int64_t m = (int64_t)k * a.ld_kmul + base;
-        cpx<R> x = {R(0), R(0)};
-        if (m < a.ld_limit) {
-          x = gload(src + (int64_t)k * a.iks, nt);
-          if (pre) x = cmul(x, pre[m]);
-        }
-        v[i] = x;
+        const bool ok = (FULL || k < (uint32_t)n) && m < a.ld_limit;
+        x[j] = {R(0), R(0)};
+        w[j] = {R(1), R(0)};
+        if (ok) x[j] = gload(src + (int64_t)k * a.iks, nt);
+        if (ok && pre) w[j] = pre[m];
+      }
+#pragma unroll
+      for (int j = 0; j < HG; ++j) {
+        const uint32_t k = k0 + (g + j) * T;
+        if (FULL || k < (uint32_t)n) v[g + j] = pre ? cmul(x[j], w[j]) : x[j];
       }
     }
   }
 }
 
-// Stores elements k = k0 + i*T of one line from v with the store hooks.
-template <typename R, int SIGN, int E = EMAX>
+// Stores elements k = k0 + i*T of one line from v with the store hooks
+// (FULL as for load_line).
+template <typename R, int SIGN, int E = EMAX, bool FULL = false>
 __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o, uint32_t k0,
                                            const cpx<R> (&v)[E]) {
   const int n = a.n;
@@ -386,7 +411,7 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) {
+      if (FULL || k < (uint32_t)n) {
         cpx<R> x = cmul_dir<SIGN>(v[i], cpx<R>{(R)w.x, cs * (R)w.y});
         gstore(dst + (int64_t)k * a.oks, x, nt);
       }
@@ -396,23 +421,32 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) {
+      if (FULL || k < (uint32_t)n) {
         cpx<R> x = v[i];
         gstore(dst + (int64_t)k * a.oks, x, nt);
       }
     }
   } else {
+    // As in load_line: a group's table loads are all issued before the
+    // first of its stores needs one.
     const int64_t base = o.st_base;
+    constexpr int HG = hook_group<R, E>();
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const uint32_t k = k0 + i * T;
-      if (k < (uint32_t)n) {
+    for (int g = 0; g < E; g += HG) {
+      cpx<R> w[HG];
+#pragma unroll
+      for (int j = 0; j < HG; ++j) {
+        const uint32_t k = k0 + (g + j) * T;
         const int64_t m = (int64_t)k * a.st_kmul + base;
-        if (m < a.st_limit) {
-          cpx<R> x = v[i];
-          if (post) x = cmul(x, post[m]);
-            gstore(dst + (int64_t)k * a.oks, x, nt);
-        }
+        w[j] = {R(1), R(0)};
+        if ((FULL || k < (uint32_t)n) && m < a.st_limit && post) w[j] = post[m];
+      }
+#pragma unroll
+      for (int j = 0; j < HG; ++j) {
+        const uint32_t k = k0 + (g + j) * T;
+        const int64_t m = (int64_t)k * a.st_kmul + base;
+        if ((FULL || k < (uint32_t)n) && m < a.st_limit)
+          gstore(dst + (int64_t)k * a.oks, post ? cmul(v[g + j], w[j]) : v[g + j], nt);
       }
     }
   }
@@ -433,9 +467,23 @@ __device__ __forceinline__ const cpx<R>* stage_twiddles_to_lds(const sa_pass_t& 
   return ltw;  // read after the next barrier
 }
 
+// Bluestein filter of line b (per-line offset when blu_level >= 0).
+template <typename R>
+__device__ __forceinline__ const cpx<R>* filter_of(const sa_pass_t& a, uint32_t b, bool active) {
+  const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
+  if (a.blu_level >= 0 && active && b < (uint32_t)a.batch) {
+    const uint32_t q2 = fdiv_d(b, a.n2div);
+    const uint32_t d2 = b - q2 * (uint32_t)a.n2;
+    const uint32_t d0 = fdiv_d(q2, a.n1div);
+    const uint32_t d1 = q2 - d0 * (uint32_t)a.n1;
+    h += (int64_t)sel3(a.blu_level, d0, d1, d2) * a.blu_hstride;
+  }
+  return h;
+}
+
 // One tile (C consecutive lines) of a pass, run by the whole workgroup.
 // Threads t >= C*T of a larger workgroup idle through the barriers.
-// TWL: the stages read their twiddles from LDS (a.tw_lds > 0, not Bluestein).
+// TWL: the stages read their twiddles from LDS (a.tw_lds > 0).
 template <typename R, int SIGN, int MODE, int MX = 0, bool TWL = false, int E = EMAX>
 __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx<R>* lds) {
   static_assert(E == EMAX || MODE != MODE_REGS, "register stages hold EMAX elements");
@@ -457,7 +505,6 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   const uint32_t tl = active ? slot_elem(a, t, id_kfast) : (uint32_t)n;
   cpx<R>* line = lds + (size_t)c * a.ldsstride;
   cpx<R> v[E];
-  static_assert(!(TWL && BLUE), "Bluestein passes read their twiddles from global memory");
   const cpx<R>* twb = reinterpret_cast<const cpx<R>*>(a.tw);
 
   // ---- load ----------------------------------------------------------------
@@ -465,7 +512,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
     const uint32_t b = line0 + c;
     if (active && b < (uint32_t)a.batch) {
       const LineOff o = line_offsets(a, b);
-      load_line<R>(a, o, tl, v);
+      load_line<R, E, true>(a, o, tl, v);
     } else {
 #pragma unroll
       for (int i = 0; i < E; ++i) v[i] = {R(0), R(0)};
@@ -499,21 +546,10 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
     const int s0 = reg_first ? 1 : 0;
     const int s1 = reg_last ? uni(a.nstages) - 1 : uni(a.nstages);
     for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX, E>(a, s, line, stage_lane(tl), lds, twb);
-    if constexpr (reg_last) last_dispatch<R, SIGN>(a, v, line, (int)tl, active, twb);
   } else {
     const int nst = uni(a.nstages);
     for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0, E>(a, s, line, stage_lane(tl), lds, twb);
-    const cpx<R>* __restrict__ h = reinterpret_cast<const cpx<R>*>(a.blu_h);
-    if (a.blu_level >= 0 && active) {
-      const uint32_t b = line0 + c;
-      if (b < (uint32_t)a.batch) {
-        const uint32_t q2 = fdiv_d(b, a.n2div);
-        const uint32_t d2 = b - q2 * (uint32_t)a.n2;
-        const uint32_t d0 = fdiv_d(q2, a.n1div);
-        const uint32_t d1 = q2 - d0 * (uint32_t)a.n1;
-        h += (int64_t)sel3(a.blu_level, d0, d1, d2) * a.blu_hstride;
-      }
-    }
+    const cpx<R>* __restrict__ h = filter_of<R>(a, line0 + c, active);
     for (int k = (int)tl; k < n; k += a.T) line[lp(k)] = cmul(line[lp(k)], h[k]);
     __syncthreads();
     for (int s = 0; s < nst; ++s) stage_dispatch<R, +1, 0, E>(a, s, line, stage_lane(tl), lds, twb);
@@ -521,10 +557,15 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
 
   // ---- store -----------------------------------------------------------------
   if constexpr (reg_last) {
+    // The last stage's outputs go to their own array: the loaded values in v
+    // are dead after the first stage and must not stay live across the
+    // stages (they did when both shared v, and parts of v went to scratch).
+    cpx<R> w[E];
+    last_dispatch<R, SIGN>(a, w, line, (int)tl, active, twb);
     const uint32_t b = line0 + c;
     if (active && b < (uint32_t)a.batch) {
       const LineOff o = line_offsets(a, b);
-      store_line<R, SIGN>(a, o, tl, v);
+      store_line<R, SIGN, E, true>(a, o, tl, w);
     }
   } else {
     const uint32_t sc = slot_line(a, t, a.st_kfast != 0);

@@ -138,7 +138,6 @@ void check_shape(const sa_pass_t& p) {
   if (((int64_t)p.C * p.ldsstride + p.tw_lds) * (int64_t)sizeof(C<R>) > 160 * 1024) die("lds bytes");
   if (p.tw_lds < 0) die("tw_lds");
   if (p.tw_lds > 0) {
-    if (p.blu_h) die("staged twiddles on a Bluestein pass");
     int64_t need = 0;  // the staged table must cover every twiddle a stage reads
     for (int s = 1; s < p.nstages; ++s)
       need = std::max<int64_t>(need, (int64_t)p.twoff[s] + (int64_t)p.ns[s] * (p.radix[s] - 1));
