@@ -22,7 +22,8 @@ Also reported, on the same JSON line:
     timed separately plus a 1-thread forward, with the oracle's radix-2 FFT
     as a secondary figure.
 
-Other workloads: --workload fft1d | batch | serve.
+Other workloads: --workload fft1d | batch | serve | serve-passes (per-pass rates
+of the serving stream's largest jobs: --prec c2c|z2z|any, --count jobs).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
                        [--no-serving] [--dims 512,512,512] [--prec z2z|c2c]
@@ -579,15 +580,74 @@ def bench_serve(args, hip):
     print(json.dumps(out), flush=True)
 
 
+def bench_serve_passes(args, hip):
+    """Per-pass HBM rates of the serving stream's largest distinct jobs: each
+    pass of each job launched `steps` times back to back between HIP events
+    on its own stream (algorithmic bytes 2 * lines * n * sizeof(T) per pass).
+    Reports every pass and the totals per precision (all passes, and the
+    generic-kernel passes alone)."""
+    from shafft_amd import api
+    from shafft_amd.workloads import draw_stream
+
+    reqs = [r for r in draw_stream(args.requests, seed=46, rate=args.rate) if args.prec == "any" or
+            r.prec == (api.Z2Z if args.prec == "z2z" else api.C2C)]
+    reqs.sort(key=lambda r: -r.elems * (16 if r.prec else 8))
+    jobs, seen = [], set()
+    for r in reqs:
+        key = (r.kind, r.n, tuple(r.dims or ()), r.prec)
+        if key not in seen:
+            seen.add(key)
+            jobs.append(r)
+        if len(jobs) >= args.count:
+            break
+    stream = hip.stream()
+    e0, e1 = hip.event(), hip.event()
+    cap = max(r.elems for r in jobs) * 2  # complex128 elements (Bluestein-free: 7-smooth sizes)
+    a, b = api.DeviceBuffer(cap, api.Z2Z), api.DeviceBuffer(cap, api.Z2Z)
+    rows, tot = [], {}
+    for r in jobs:
+        p = (api.OneDBackendPlan(r.n, r.prec) if r.kind == 1 else
+             api.NDBackendPlan(list(r.dims), list(range(len(r.dims))), r.prec))
+        p.set_stream(stream)
+        desc = p.describe().strip().splitlines()
+        elems = cap * 16 // (16 if r.prec else 8)
+        for i in range(p.num_passes()):
+            p.execute_pass(i, a.ptr, elems, b.ptr, elems, r.direction)
+            hip.record(e0, stream)
+            for _ in range(args.steps):
+                p.execute_pass(i, a.ptr, elems, b.ptr, elems, r.direction)
+            hip.record(e1, stream)
+            ms = hip.elapsed_ms(e0, e1) / args.steps
+            by = p.pass_bytes(i)
+            generic = "fixed" not in desc[i]
+            t = tot.setdefault(r.prec, [0, 0.0, 0, 0.0])
+            t[0] += by
+            t[1] += ms
+            if generic:
+                t[2] += by
+                t[3] += ms
+            rows.append({"job": r.describe(), "pass": desc[i], "us": round(ms * 1e3, 1),
+                         "gbs": round(by / (ms * 1e-3) / 1e9, 1)})
+    for row in rows:
+        print(f"{row['job']:38s} {row['pass'][:100]:100s} {row['us']:8.1f} us {row['gbs']:8.1f} GB/s", flush=True)
+    out = {"metric": "per-pass HBM rate of the serving stream's largest jobs (GB/s, algorithmic bytes)",
+           "unit": "GB/s", "jobs": len(jobs), "steps": args.steps}
+    for pr, t in sorted(tot.items()):
+        name = "complex128" if pr else "complex64"
+        out[name] = {"all_passes_gbs": round(t[0] / (t[1] * 1e-3) / 1e9, 1),
+                     "generic_passes_gbs": round(t[2] / max(t[3] * 1e-3, 1e-12) / 1e9, 1)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--dims", type=str, default="512,512,512")
-    ap.add_argument("--prec", type=str, default="z2z", choices=["c2c", "z2z"])
+    ap.add_argument("--prec", type=str, default="z2z", choices=["c2c", "z2z", "any"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="fftnd", choices=["fftnd", "batch", "serve", "fft1d"])
+    ap.add_argument("--workload", default="fftnd", choices=["fftnd", "batch", "serve", "fft1d", "serve-passes"])
     ap.add_argument("--n", type=int, default=1 << 27, help="fft1d length")
     ap.add_argument("--requests", type=int, default=2000, help="jobs in the serve workload")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve arrival rate (jobs/s)")
@@ -619,6 +679,9 @@ def main():
         return
     if args.workload == "fft1d":
         bench_fft1d(args, Hip())
+        return
+    if args.workload == "serve-passes":
+        bench_serve_passes(args, Hip())
         return
 
     from shafft_amd import api
