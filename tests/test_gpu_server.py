@@ -143,3 +143,78 @@ def test_served_round_trip(gpu_lib):
         y = api.download_ptr(c.ptr, x.size, r.prec).reshape(shape)
         assert oracle.check_rel_error(y, x, TOL[r.prec]), (r.describe(), oracle.rel_l2(y, x))
     srv.close()
+
+
+def _plain(r, x):
+    """The plain backend call on a copy of x (execute may overwrite its input)."""
+    shape = (r.n,) if r.kind == 1 else tuple(r.dims)
+    if r.kind == 1:
+        p = api.OneDBackendPlan(r.n, r.prec)
+        a, b = api.DeviceBuffer(x.size, r.prec), api.DeviceBuffer(x.size, r.prec)
+        a.upload(x.ravel())
+        p.execute(a.ptr, b.ptr, r.direction)
+        p.synchronize()
+        return api.download_ptr(b.ptr, x.size, r.prec).reshape(shape)
+    p = api.NDBackendPlan(list(r.dims), list(range(len(r.dims))), r.prec)
+    a, b = api.DeviceBuffer(x.size, r.prec), api.DeviceBuffer(x.size, r.prec)
+    a.upload(x.ravel())
+    d, _ = p.execute(a.ptr, b.ptr, r.direction)
+    return api.download_ptr(d, x.size, r.prec).reshape(shape)
+
+
+def test_served_c5_stream_paced(gpu_lib):
+    """A draw of the C5 serving stream (Poisson arrivals at 2000 jobs/s,
+    sizes 2^10..2^26, 70% complex64) served at its arrival times, with the
+    2^25..2^26-element jobs of both precisions in it: every result bit for bit
+    the plain backend call on the same input, and every job's inverse
+    (served too, normalised) back to its input within TOL."""
+    import time
+
+    reqs = draw_stream(600, seed=46)
+    big = {p: [r for r in reqs if r.prec == p and r.elems >= 1 << 25][:3] for p in (0, 1)}
+    assert all(len(v) >= 2 for v in big.values()), "stream draw lacks 2^25+ jobs of a precision"
+    jobs = sorted(big[0] + big[1] + reqs[:24], key=lambda r: r.t)
+    assert max(r.elems for r in jobs) > 60_000_000
+    xs, ins, outs = [], [], []
+    for k, r in enumerate(jobs):
+        shape = (r.n,) if r.kind == 1 else tuple(r.dims)
+        x = oracle.uniform_input(shape, 4600 + k, DT[r.prec])
+        b = api.DeviceBuffer(x.size, r.prec)
+        b.upload(x.ravel())
+        xs.append(x)
+        ins.append(b)
+        outs.append(api.DeviceBuffer(x.size, r.prec))
+    srv = api.Server(slots=8, arena_bytes=24 << 30, planners=2)
+    # plans first, so the paced phase runs from resident plans
+    for r, i, o in zip(jobs, ins, outs):
+        sub = srv.submit_1d if r.kind == 1 else srv.submit_nd
+        sub(r.n if r.kind == 1 else r.dims, r.prec, r.direction, i.ptr, o.ptr)
+    srv.drain()
+    t0 = api.Server.now_ns() + 2_000_000
+    base = jobs[0].t
+    ids = []
+    for r, i, o in zip(jobs, ins, outs):
+        arr = t0 + int((r.t - base) * 1e9)
+        while time.monotonic_ns() < arr:
+            pass
+        sub = srv.submit_1d if r.kind == 1 else srv.submit_nd
+        ids.append(sub(r.n if r.kind == 1 else r.dims, r.prec, r.direction, i.ptr, o.ptr, arrival_ns=arr))
+    res = [srv.wait(j, 120) for j in ids]
+    for r, s in zip(jobs, res):
+        assert s is not None and s.status == 0, (r.describe(), s and s.message)
+    lat = sorted(s.latency_ms for s in res)
+    print(f"paced C5 draw: {len(jobs)} jobs, largest {max(r.elems for r in jobs)} elements, "
+          f"p50 {lat[len(lat) // 2]:.3f} ms, max {lat[-1]:.3f} ms", flush=True)
+    backs = [api.DeviceBuffer(x.size, r.prec) for x, r in zip(xs, jobs)]
+    inv = []
+    for r, o, b in zip(jobs, outs, backs):
+        sub = srv.submit_1d if r.kind == 1 else srv.submit_nd
+        inv.append(sub(r.n if r.kind == 1 else r.dims, r.prec, 1 - r.direction, o.ptr, b.ptr, normalize=True))
+    for j in inv:
+        assert srv.wait(j, 120).status == 0
+    srv.close()
+    for r, x, o, b in zip(jobs, xs, outs, backs):
+        y = api.download_ptr(o.ptr, x.size, r.prec).reshape(x.shape)
+        assert np.array_equal(y, _plain(r, x)), r.describe()
+        z = api.download_ptr(b.ptr, x.size, r.prec).reshape(x.shape) / np.sqrt(x.size)
+        assert oracle.check_rel_error(z, x, TOL[r.prec]), (r.describe(), oracle.rel_l2(z, x))
