@@ -855,6 +855,82 @@ void PassProgram::add_transform_impl(int64_t N, const std::vector<Digit>& lines,
   }
 }
 
+// Move a pass of the three-pass Bluestein to the compile-time-length kernels
+// of fft_blue.hip: the kernel's radix plan and stage table, C = 2^logC lines
+// per tile, one tile per workgroup.  Column passes (line-fast on both sides)
+// take 256-byte row segments (32 complex-float / 16 complex-double adjacent
+// columns); row passes take at least 256 threads.  Knobs:
+//   SHAFFT_AMD_BLUE_FX=0   keep the generic kernel
+//   SHAFFT_AMD_BLUE_COLC   columns per column tile
+//   SHAFFT_AMD_BLUE_ROWT   minimum threads per row tile
+//   SHAFFT_AMD_BLUE_TWL    stage twiddles staged in LDS: 1 always, 0 never
+//                          (default: column passes only; a row pass's table
+//                          of ~n entries would cost workgroups per CU:
+//                          1,000,003-point complex double row pass 37.2 us
+//                          with it, 26.0 us without)
+bool PassProgram::configure_blue(PassSpec& ps, int kind) {
+  sa_pass_t& p = ps.p;
+  const int n = p.n;
+  if (!env_int("SHAFFT_AMD_BLUE_FX", 1) || !sa_blue_supported(n, kind)) return false;
+  const bool cols = kind != SA_BLUE_ROWS;
+  if (cols ? (p.ld_kfast || p.st_kfast) : (!p.ld_kfast || !p.st_kfast)) return false;
+  const int esz = prec_ ? 16 : 8;
+  // Column tiles: 256-byte row segments while that leaves a tile for every
+  // CU, else narrower (not below 64 bytes): these passes are short, and a
+  // CU without a tile costs more than the partial cache lines of narrower
+  // segments (1,000,003-point complex float, 4096 columns: 16 columns,
+  // 13.1 us, against 32 columns on half the CUs, 15.5 us).
+  int C = 1;
+  if (cols) {
+    C = 256 / esz;
+    while (C > 64 / esz && (p.batch + C - 1) / C < device_cus()) C /= 2;
+    C = env_int("SHAFFT_AMD_BLUE_COLC", C);
+  } else {
+    C = std::max(1, env_int("SHAFFT_AMD_BLUE_ROWT", 256) / (n / 16));
+  }
+  const int E = sa_blue_elems(n, prec_, kind, C);
+  const int T = n / E;
+  int logC = 0;
+  while ((1 << (logC + 1)) <= C) ++logC;
+  while (logC > 0 && ((int64_t)1 << (logC - 1)) >= p.batch) --logC;
+  int stride = n + (n - 1) / 16;
+  if (stride % 2 == 0) stride += 1;
+  const int budget = lds_budget_bytes();
+  while (logC > 0 && (((int64_t)T << logC) > 512 || ((int64_t)stride << logC) * esz > budget)) --logC;
+  if (((int64_t)T << logC) > 512 || ((int64_t)stride << logC) * esz > budget) return false;
+  const std::vector<int> rad = fixed_radix_plan(n);
+  std::vector<int> offs;
+  p.tw = stage_table(rad, offs);
+  p.nstages = (int32_t)rad.size();
+  int64_t Ns = 1;
+  for (size_t s = 0; s < rad.size(); ++s) {
+    p.radix[s] = rad[s];
+    p.ns[s] = (int32_t)Ns;
+    p.twoff[s] = offs[s];
+    p.nsdiv[s] = make_fastdiv((uint32_t)Ns);
+    p.nbdiv[s] = make_fastdiv((uint32_t)(n / rad[s]));
+    Ns *= rad[s];
+  }
+  int64_t tw_elems = 0;
+  for (int s = 1; s < p.nstages; ++s)
+    tw_elems = std::max<int64_t>(tw_elems, (int64_t)p.twoff[s] + (int64_t)p.ns[s] * (p.radix[s] - 1));
+  p.tw_lds = 0;
+  if (env_int("SHAFFT_AMD_BLUE_TWL", cols ? 1 : 0) && tw_elems > 0 &&
+      (((int64_t)stride << logC) + tw_elems) * esz <= budget)
+    p.tw_lds = (int32_t)tw_elems;
+  p.logC = logC;
+  p.C = 1 << logC;
+  p.T = T;
+  p.emax = E;
+  p.ldsstride = stride;
+  p.Tdiv = make_fastdiv((uint32_t)T);
+  p.Cdiv = make_fastdiv((uint32_t)p.C);
+  p.reg_first = p.reg_last = 0;
+  p.xcd_order = 0;
+  ps.blue = kind;
+  return true;
+}
+
 // Bluestein convolution of length M = N1 * N2 in three passes over a
 // contiguous axis (iks = oks = 1), keeping the spectrum in transposed order:
 //   A: N1-point forward DFTs along n1 (element stride N2) of the chirped,
@@ -872,7 +948,7 @@ bool PassProgram::bluestein_three_pass(int64_t N, int64_t M, const std::vector<D
                                        const void* const tc[2],
                                        const std::vector<std::complex<long double>> H[2]) {
   const int64_t lfc = lf_cap(prec_);
-  int64_t N2 = std::min<int64_t>(4096, M / 16);
+  int64_t N2 = std::min<int64_t>(env_int("SHAFFT_AMD_BLUE_N2", 4096), M / 16);
   int64_t N1 = M / N2;
   if (N1 > lfc) {
     N1 = lfc;
@@ -905,6 +981,7 @@ bool PassProgram::bluestein_three_pass(int64_t N, int64_t M, const std::vector<D
   lda.level = n2a;
   lda.dmul = 1;
   emit_single(N1, la, N2, N2, -1, lda, ElemHook{}, nullptr, n2a, M, L * M);
+  configure_blue(passes_.back(), SA_BLUE_COLS_IN);
 
   std::vector<Digit> lb = lines;
   for (size_t j = 0; j < lb.size(); ++j) lb[j].is = lb[j].os = cs[j];
@@ -917,6 +994,7 @@ bool PassProgram::bluestein_three_pass(int64_t N, int64_t M, const std::vector<D
     p.blu_level = p.ptw_level;  // the k1 digit, after emit_single's remapping
     p.blu_hstride = N2;
   }
+  configure_blue(passes_.back(), SA_BLUE_ROWS);
 
   std::vector<Digit> lc = lines;
   for (size_t j = 0; j < lc.size(); ++j) lc[j].is = cs[j];
@@ -929,6 +1007,7 @@ bool PassProgram::bluestein_three_pass(int64_t N, int64_t M, const std::vector<D
   stc.level = n2c;
   stc.dmul = 1;
   emit_single(N1, lc, N2, N2, +1, ElemHook{}, stc, nullptr, -1, 0, L * N);
+  configure_blue(passes_.back(), SA_BLUE_COLS_OUT);
 
   PassSpec* ps = &passes_[first];
   for (int d = 0; d < 3; ++d) ps[d].dir_tables = true;
@@ -1003,8 +1082,9 @@ void PassProgram::run_one(const SlotMap& m, size_t i, void* const slots[SLOT_COU
     p.blu_h = ps.blu_t[d];
   }
   const int sign = ps.sign == 0 ? dir : ps.sign;
-  int rc = ps.fixed ? sa_launch_fixed(&p, prec_, sign, ps.grid, stream)
-                    : sa_launch_pass(&p, prec_, sign, stream);
+  int rc = ps.blue    ? sa_launch_blue(&p, prec_, ps.blue, stream)
+           : ps.fixed ? sa_launch_fixed(&p, prec_, sign, ps.grid, stream)
+                      : sa_launch_pass(&p, prec_, sign, stream);
   hip_check((hipError_t)rc, "fft pass launch");
 }
 
@@ -1031,6 +1111,7 @@ std::string PassProgram::describe() const {
     const sa_pass_t& p = passes_[i].p;
     os << "pass " << i << ": n=" << p.n << " batch=" << p.batch
        << (passes_[i].fixed ? " fixed(grid=" + std::to_string(passes_[i].grid) + ")" : "")
+       << (passes_[i].blue ? " blue" + std::to_string(passes_[i].blue) : "")
        << " radices=";
     for (int s = 0; s < p.nstages; ++s) os << (s ? "x" : "") << p.radix[s];
     os << " iks=" << p.iks << " oks=" << p.oks << " C=" << p.C << " T=" << p.T

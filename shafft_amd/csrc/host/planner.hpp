@@ -62,6 +62,8 @@ struct PassSpec {
   // Fixed-length persistent kernel and its grid (workgroups).
   bool fixed = false;
   int grid = 0;
+  // Three-pass Bluestein kernel kind (SA_BLUE_*), 0: none.
+  int blue = 0;
   // Bluestein passes carry direction-dependent tables: [0] forward, [1] backward.
   bool dir_tables = false;
   const void* pre_t[2] = {nullptr, nullptr};
@@ -140,6 +142,7 @@ class PassProgram {
   bool bluestein_three_pass(int64_t N, int64_t M, const std::vector<Digit>& lines,
                             const void* const tc[2],
                             const std::vector<std::complex<long double>> H[2]);
+  bool configure_blue(PassSpec& ps, int kind);
   bool lf_split_ = true;  // split strided lines at lf_cap (else only at lds_cap)
   std::vector<DevBuf> tables_;
   std::map<std::vector<int>, std::pair<const void*, std::vector<int>>> stage_cache_;

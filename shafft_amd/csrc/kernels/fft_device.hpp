@@ -65,6 +65,27 @@ __device__ __forceinline__ cpx<R> crot(cpx<R> a, R c, R s) {
   return {a.x * c - a.y * ss, a.x * ss + a.y * c};
 }
 
+#ifdef __HIPCC__
+// Copies n complex values from global memory into LDS with the whole
+// workgroup.  Each thread issues U loads before its first LDS store (a plain
+// strided loop waits for every load in turn: one L2 round trip per element
+// per thread); out-of-range slots load element n-1 and are not stored, so no
+// load is predicated.
+template <typename R, int U = 8>
+__device__ __forceinline__ void copy_to_lds(cpx<R>* __restrict__ dst, const cpx<R>* __restrict__ src,
+                                            int n) {
+  const int nt = (int)blockDim.x;
+  for (int base = (int)threadIdx.x; base < n; base += U * nt) {
+    cpx<R> v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = src[min(base + u * nt, n - 1)];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + u * nt < n) dst[base + u * nt] = v[u];
+  }
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // Butterflies: in-register DFT of length RAD, X[q] = sum_r v[r] w_RAD^{SIGN r q}.
 // ---------------------------------------------------------------------------

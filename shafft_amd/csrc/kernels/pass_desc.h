@@ -109,6 +109,18 @@ int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid, void* stre
 /* Resident workgroups per CU of that kernel for a block size and LDS size. */
 int sa_fixed_occupancy(int n, int prec, int sign, int nthr, size_t lds);
 
+/* Three-pass Bluestein kernels on compile-time power-of-two lengths
+ * (fft_blue.hip): the chirped, zero-padded column pass, the row pass
+ * (forward DFT, per-row filter, inverse DFT, conjugate four-step twiddle)
+ * and the chirped, truncating inverse column pass.  Same descriptor
+ * semantics as the generic pass; tiles of C = 2^logC lines, one tile per
+ * workgroup, T = n / sa_blue_elems(...) threads per line (p->emax). */
+enum { SA_BLUE_COLS_IN = 1, SA_BLUE_ROWS = 2, SA_BLUE_COLS_OUT = 3 };
+int sa_blue_supported(int n, int kind);
+/* Elements per thread of that kernel for n-point lines in tiles of `lines`. */
+int sa_blue_elems(int n, int prec, int kind, int lines);
+int sa_launch_blue(const sa_pass_t* p, int prec, int kind, void* stream);
+
 /* Batched persistent launch: many passes of many transforms in one grid.
  * Workgroups pull items (pass, tile) from a device queue in list order;
  * an item waits until every tile of the pass it depends on has completed.

@@ -99,6 +99,7 @@ void stages(const sa_pass_t& p, std::vector<C<R>>& d, int sign) {
 }
 
 thread_local bool g_fixed_launch = false;
+thread_local int g_blue_emax = 0;  // elements per thread of a Bluestein-kernel launch
 
 template <typename R>
 void check_shape(const sa_pass_t& p) {
@@ -107,7 +108,8 @@ void check_shape(const sa_pass_t& p) {
     std::abort();
   };
   // the fixed kernel holds 32 complex floats per thread at n = 512, 1024
-  const int emax = g_fixed_launch && sizeof(R) == 4 && (p.n == 512 || p.n == 1024) ? 32 : 16;
+  const int emax = g_blue_emax ? g_blue_emax
+                               : (g_fixed_launch && sizeof(R) == 4 && (p.n == 512 || p.n == 1024) ? 32 : 16);
   if (p.T < 1 || p.C < 1 || p.C * p.T > 512) die("threads per workgroup");
   if ((int64_t)p.n > (int64_t)emax * p.T) die("n > EMAX * T");
   if (!g_fixed_launch) {
@@ -309,6 +311,70 @@ extern "C" int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid,
   g_fixed_launch = true;
   const int rc = sa_launch_pass(&q, prec, sign, nullptr);
   g_fixed_launch = false;
+  return rc;
+}
+
+extern "C" int sa_blue_elems(int n, int prec, int kind, int lines) {
+  return (kind != SA_BLUE_ROWS && prec == 0 && n == 512 && lines > 16) ? 32 : 16;
+}
+
+extern "C" int sa_blue_supported(int n, int kind) {
+  if (kind == SA_BLUE_ROWS) return n == 1024 || n == 2048 || n == 4096 || n == 8192;
+  if (kind == SA_BLUE_COLS_IN || kind == SA_BLUE_COLS_OUT)
+    return n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
+  return 0;
+}
+
+// The three-pass Bluestein kernels (fft_blue.hip): the generic semantics,
+// after the checks of what those kernels assume (grid = tiles, fixed radix
+// plan, lane mapping per kind, hooks per kind).
+extern "C" int sa_launch_blue(const sa_pass_t* p, int prec, int kind, void*) {
+  auto die = [](const char* m) {
+    std::fprintf(stderr, "EMU BAD BLUESTEIN LAUNCH: %s\n", m);
+    std::abort();
+  };
+  if (!sa_blue_supported(p->n, kind)) die("length");
+  if (p->emax != sa_blue_elems(p->n, prec, kind, p->C)) die("elements per thread");
+  if (p->T * p->emax != p->n) die("T");
+  if (p->C != (1 << p->logC)) die("C");
+  if (p->C * p->T > 512) die("threads");
+  if (((int64_t)p->C * p->ldsstride + p->tw_lds) * (prec ? 16 : 8) > 160 * 1024) die("lds");
+  if (p->ldsstride < p->n + (p->n - 1) / 16) die("stride");
+  if (p->tw_lds > 0) {
+    int64_t need = 0;
+    for (int s = 1; s < p->nstages; ++s)
+      need = std::max<int64_t>(need, (int64_t)p->twoff[s] + (int64_t)p->ns[s] * (p->radix[s] - 1));
+    if (p->tw_lds < need) die("tw_lds below the stage table");
+    chk("tw", p->tw, p->tw_lds - 1, prec ? 16 : 8);
+  }
+  {
+    static const std::map<int, std::vector<int>> plans = {
+        {32, {8, 4}},       {64, {8, 8}},       {128, {16, 8}},     {256, {16, 16}},
+        {512, {8, 8, 8}},   {1024, {16, 8, 8}}, {2048, {16, 16, 8}}, {4096, {16, 16, 16}},
+        {8192, {16, 16, 8, 4}}};
+    const auto& pl = plans.at(p->n);
+    if ((int)pl.size() != p->nstages) die("stage count differs from the kernel's plan");
+    for (int s = 0; s < p->nstages; ++s)
+      if (pl[s] != p->radix[s]) die("radix differs from the kernel's plan");
+  }
+  if (kind == SA_BLUE_ROWS) {
+    if (!p->ld_kfast || !p->st_kfast) die("row pass must be k-fast");
+    if (!p->blu_h || p->pre || p->post || p->ptw_level < 0 || !p->ptw_conj) die("row pass hooks");
+  } else {
+    if (p->ld_kfast || p->st_kfast) die("column pass must be line-fast");
+    if (p->blu_h || p->blu_level >= 0) die("column pass filter");
+    if (kind == SA_BLUE_COLS_IN && (!p->pre || p->post || p->st_level >= 0 || p->ptw_level < 0))
+      die("chirp-in hooks");
+    if (kind == SA_BLUE_COLS_OUT && (p->pre || !p->post || p->ld_level >= 0 || p->ptw_level >= 0))
+      die("chirp-out hooks");
+  }
+  sa_pass_t q = *p;
+  q.reg_first = q.reg_last = 0;
+  g_fixed_launch = true;
+  g_blue_emax = p->emax;
+  const int rc = sa_launch_pass(&q, prec, kind == SA_BLUE_COLS_OUT ? +1 : -1, nullptr);
+  g_fixed_launch = false;
+  g_blue_emax = 0;
   return rc;
 }
 

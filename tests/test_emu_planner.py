@@ -109,6 +109,31 @@ def test_long_bluestein_three_pass(emu_lib):
             assert oracle.check_rel_error(y, ref, TOL[prec]), (dims, d, oracle.rel_l2(y, ref))
 
 
+def test_bluestein_fixed_kernels(emu_lib):
+    """Three-pass Bluestein on the compile-time-length kernels (column
+    passes of 32..512 points with the chirp hooks, row passes of 1024..8192
+    points with the filter): the emulation checks each launch against what
+    those kernels assume, and the results against numpy's FFT (pinned
+    against the direct DFT at primes in test_oracle), with outer lines."""
+    L = emu_lib
+    for dims, prec, kinds in [((40009,), 0, ("blue1", "blue2", "blue3")), ((2, 40009), 1, ("blue1", "blue2", "blue3")),
+                              ((3, 12007), 0, ("blue2",)), ((70001,), 1, ("blue1", "blue2", "blue3"))]:
+        x = oracle.uniform_input(dims, 13, DT[prec])
+        p = api.NDBackendPlan(list(dims), [len(dims) - 1], prec, L=L)
+        desc = p.describe()
+        assert p.num_passes() == 3, desc
+        for k in kinds:
+            assert k in desc, desc
+        a, b = api.DeviceBuffer(x.size, prec, L), api.DeviceBuffer(x.size, prec, L)
+        x128 = x.astype(np.complex128)
+        for d in (api.FORWARD, api.BACKWARD):
+            a.upload(x.ravel())
+            dd, _ = p.execute(a.ptr, b.ptr, d)
+            y = api.download_ptr(dd, x.size, prec, L).reshape(dims)
+            ref = np.fft.fft(x128, axis=-1) if d == api.FORWARD else np.fft.ifft(x128, axis=-1) * dims[-1]
+            assert oracle.check_rel_error(y, ref, TOL[prec]), (dims, d, oracle.rel_l2(y, ref))
+
+
 def test_1d_plans_small_cap_multilevel(emu_lib):
     # A tiny LDS cap forces three-level four-step and three-level Bluestein.
     out = _run_in_subprocess(SUB.replace("CASES", "[(4096,1),(32768,1),(1000,0),(997,1),(30030,1),(6561,1)]"),
