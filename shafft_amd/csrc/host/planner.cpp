@@ -60,10 +60,14 @@ int64_t lds_cap(int prec) {
 // Longest line for a pass whose elements are strided in memory.  Such a pass
 // reads (or writes) C adjacent lines per element index, so its HBM segments
 // are C * element-size bytes; with 16 elements per thread and 512 threads
-// per workgroup, C >= 16 needs n <= 512.  Longer strided lines are split
-// (four-step) rather than run with fewer, narrower lines.
+// per workgroup, C >= 16 needs n <= 512.  Lines up to 1024 still run as one
+// pass with fewer, narrower lines: splitting them costs a whole extra pass,
+// and one pass at 64-byte segments beats two at 128 (serving-stream jobs
+// with a 525..675-point strided axis or a 625-point four-step factor:
+// 4 -> 3 passes, 12-20 % less time per job).  Longer ones are split
+// (four-step).
 int64_t lf_cap(int prec) {
-  int64_t c = 512;
+  int64_t c = 1024;
   if (const char* e = std::getenv("SHAFFT_AMD_LF_CAP")) {
     const long long v = std::atoll(e);
     if (v >= 8) c = v;
@@ -362,6 +366,13 @@ static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
   const int thr_dflt = 65536 / (emax * elem_bytes);
   const int max_thr = std::max(T, std::min(512, env_int("SHAFFT_AMD_WG_THREADS", thr_dflt)));
   while (C > 1 && C * T > max_thr) C /= 2;
+  // A line-fast pass takes as many lines as the thread budget holds, not
+  // the power of two below it: wider row segments (405-point complex float:
+  // 19 lines, 152 bytes, instead of 16) and long strided lines past 512
+  // points (625: 12 lines instead of 8); the serving stream's largest jobs
+  // 12.5 -> 11.8 ms summed (SHAFFT_AMD_LF_CFIT=0: powers of two).
+  if (lf && env_int("SHAFFT_AMD_LF_CFIT", 1))
+    C = (int)std::min<int64_t>(std::max(C, max_thr / T), p.n2);
   // padded row: index k -> k + k/16, odd stride so line-fast lanes spread
   int stride = n + (n - 1) / 16;
   if (stride % 2 == 0) stride += 1;
@@ -947,7 +958,9 @@ bool PassProgram::configure_blue(PassSpec& ps, int kind) {
 bool PassProgram::bluestein_three_pass(int64_t N, int64_t M, const std::vector<Digit>& lines,
                                        const void* const tc[2],
                                        const std::vector<std::complex<long double>> H[2]) {
-  const int64_t lfc = lf_cap(prec_);
+  // Column passes of at most 512 points (16 adjacent columns of the
+  // chirped input per tile at the least).
+  const int64_t lfc = std::min<int64_t>(lf_cap(prec_), 512);
   int64_t N2 = std::min<int64_t>(env_int("SHAFFT_AMD_BLUE_N2", 4096), M / 16);
   int64_t N1 = M / N2;
   if (N1 > lfc) {

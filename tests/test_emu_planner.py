@@ -162,14 +162,16 @@ for dims in [(64, 48, 30), (7, 1000), (3, 257, 2)]:
 
 
 def test_nd_strided_axis_split(emu_lib):
-    """A strided axis longer than the strided-line cap (512) is split
+    """A strided axis longer than the strided-line cap (1024) is split
     (four-step, or a four-pass Bluestein chain for a prime), with the
     unit-stride row digit kept innermost so every pass is line-fast (C
-    adjacent rows per element) on both sides.  (Axis blocks that do not
-    include the last axis: those keep the axis-by-axis plan.)"""
+    adjacent rows per element) on both sides; strided axes up to the cap
+    stay one pass.  (Axis blocks that do not include the last axis: those
+    keep the axis-by-axis plan.)"""
     L = emu_lib
-    for dims, axes, prec, npass in [((1200, 6), [0], 0, 2), ((6, 700, 5), [0, 1], 1, 3),
-                                    ((1031, 4), [0], 1, 4), ((600, 22), [0], 0, 2)]:
+    for dims, axes, prec, npass in [((1200, 6), [0], 0, 2), ((6, 1400, 5), [0, 1], 1, 3),
+                                    ((1031, 4), [0], 1, 4), ((2100, 22), [0], 0, 2),
+                                    ((6, 700, 5), [0, 1], 1, 2), ((600, 22), [0], 0, 1)]:
         x = oracle.uniform_input(dims, 7, DT[prec])
         p = api.NDBackendPlan(dims, axes, prec, L=L)
         assert p.num_passes() == npass, (dims, p.describe())
