@@ -60,6 +60,11 @@ __device__ __forceinline__ void gstore(cpx<R>* p, cpx<R> x, bool nt) {
 // ---------------------------------------------------------------------------
 
 // LDS -> LDS, butterflies jb = tl + i*T held in registers across a barrier.
+// Every slot computes a butterfly: a slot past the line's last butterfly
+// (jb >= nb) computes the last one again and only its writes are skipped,
+// so the reads, twiddles and DFT run without per-butterfly branches (a
+// guarded butterfly cost exec-mask branches and register moves at every
+// join).  Ns is wave-uniform (the twiddle branch is a scalar branch).
 template <typename R, int RAD, int SIGN, int E = EMAX>
 __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdiv_t nsd,
                                           const cpx<R>* __restrict__ tw, int tl, int T) {
@@ -67,32 +72,28 @@ __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdi
   const int nb = n / RAD;
   const FastDiv fd{nsd.d, nsd.m, nsd.s};
   cpx<R> v[MAXB][RAD];
+  int idx[MAXB];
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
-    const int jb = tl + i * T;
-    if (jb < nb) {
+    const int jb = min(tl + i * T, nb - 1);
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) v[i][r] = line[lp(jb + r * nb)];
-      if (Ns > 1) {
-        const int q = (int)fdiv((uint32_t)jb, fd);
-        const int rem = jb - q * Ns;
-        const cpx<R>* w = tw + rem * (RAD - 1);
+    for (int r = 0; r < RAD; ++r) v[i][r] = line[lp(jb + r * nb)];
+    const int q = (int)fdiv((uint32_t)jb, fd);
+    const int rem = jb - q * Ns;
+    if (Ns > 1) {
+      const cpx<R>* w = tw + rem * (RAD - 1);
 #pragma unroll
-        for (int r = 1; r < RAD; ++r) v[i][r] = cmul_dir<SIGN>(v[i][r], w[r - 1]);
-      }
-      dft<RAD, SIGN>(v[i]);
+      for (int r = 1; r < RAD; ++r) v[i][r] = cmul_dir<SIGN>(v[i][r], w[r - 1]);
     }
+    dft<RAD, SIGN>(v[i]);
+    idx[i] = q * Ns * RAD + rem;
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
-    const int jb = tl + i * T;
-    if (jb < nb) {
-      const int q = (int)fdiv((uint32_t)jb, fd);
-      const int rem = jb - q * Ns;
-      const int idx = q * Ns * RAD + rem;
+    if (tl + i * T < nb) {
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) line[lp(idx + r * Ns)] = v[i][r];
+      for (int r = 0; r < RAD; ++r) line[lp(idx[i] + r * Ns)] = v[i][r];
     }
   }
   __syncthreads();
@@ -249,7 +250,7 @@ __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>
       return;
     }
   }
-  stage_run<R, SIGN, E>(uni(a.radix[s]), a.n, a.ns[s], a.nsdiv[s], a.nbdiv[s], tws, line, tl, a.T,
+  stage_run<R, SIGN, E>(uni(a.radix[s]), a.n, uni(a.ns[s]), a.nsdiv[s], a.nbdiv[s], tws, line, tl, a.T,
                      lds, a.ldsstride, a.C, stage_lane(threadIdx.x));
 }
 
@@ -346,10 +347,13 @@ __device__ __forceinline__ void load_line(const sa_pass_t& a, const LineOff& o, 
   const bool nt = a.nt != 0;
   const bool plain = a.pre == nullptr && a.ld_level < 0 && a.ld_kmul == 1 && a.ld_limit >= n;
   if (plain) {
+    // Slots past the line's end load its last element (same address as a
+    // live lane, no extra traffic) instead of being predicated; the caller
+    // drops them.
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const uint32_t k = k0 + i * T;
-      if (FULL || k < (uint32_t)n) v[i] = gload(src + (int64_t)k * a.iks, nt);
+      const uint32_t k = FULL ? k0 + i * T : min(k0 + i * T, (uint32_t)n - 1u);
+      v[i] = gload(src + (int64_t)k * a.iks, nt);
     }
   } else {
     // Hooked elements in groups of HG: every data and table load of a group
@@ -411,10 +415,8 @@ __device__ __forceinline__ void store_line(const sa_pass_t& a, const LineOff& o,
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t k = k0 + i * T;
-      if (FULL || k < (uint32_t)n) {
-        cpx<R> x = cmul_dir<SIGN>(v[i], cpx<R>{(R)w.x, cs * (R)w.y});
-        gstore(dst + (int64_t)k * a.oks, x, nt);
-      }
+      const cpx<R> x = cmul_dir<SIGN>(v[i], cpx<R>{(R)w.x, cs * (R)w.y});
+      if (FULL || k < (uint32_t)n) gstore(dst + (int64_t)k * a.oks, x, nt);
       if (i + 1 < E) w = cmul(w, step);
     }
   } else if (post == nullptr && a.st_level < 0 && a.st_kmul == 1 && a.st_limit >= n) {
@@ -463,7 +465,7 @@ template <typename R>
 __device__ __forceinline__ const cpx<R>* stage_twiddles_to_lds(const sa_pass_t& a, cpx<R>* lds) {
   const cpx<R>* __restrict__ g = reinterpret_cast<const cpx<R>*>(a.tw);
   cpx<R>* ltw = lds + (size_t)a.C * a.ldsstride;
-  for (int i = (int)threadIdx.x; i < a.tw_lds; i += (int)blockDim.x) ltw[i] = g[i];
+  copy_to_lds<R>(ltw, g, a.tw_lds);
   return ltw;  // read after the next barrier
 }
 
@@ -575,10 +577,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
       const cpx<R>* ll = lds + (size_t)sc * a.ldsstride;
       cpx<R> u[E];
 #pragma unroll
-      for (int i = 0; i < E; ++i) {
-        const uint32_t k = k0 + i * (uint32_t)a.T;
-        if (k < (uint32_t)n) u[i] = ll[lp((int)k)];
-      }
+      for (int i = 0; i < E; ++i) u[i] = ll[lp((int)min(k0 + i * (uint32_t)a.T, (uint32_t)n - 1u))];
       const LineOff o = line_offsets(a, b);
       store_line<R, SIGN, E>(a, o, k0, u);
     }
