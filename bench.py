@@ -639,6 +639,43 @@ def bench_serve_passes(args, hip):
     print(json.dumps(out), flush=True)
 
 
+def bench_job_pass(args, hip):
+    """One pass of one job (FFT1D of --n points, or FFTND of --dims over every
+    axis), launched `steps` times back to back between HIP events: the unit a
+    counter run (rocprofv3 --pmc) is taken on."""
+    from shafft_amd import api
+
+    prec = api.Z2Z if args.prec == "z2z" else api.C2C
+    if args.dims_nd:
+        dims = [int(v) for v in args.dims_nd.split(",")]
+        p = api.NDBackendPlan(dims, list(range(len(dims))), prec)
+        elems = int(np.prod(dims))
+        name = "FFTND " + "x".join(map(str, dims))
+    else:
+        p = api.OneDBackendPlan(args.n, prec)
+        elems = args.n
+        name = f"FFT1D N={args.n}"
+    stream = hip.stream()
+    p.set_stream(stream)
+    ext = max(max(p.pass_extents(i)) for i in range(p.num_passes()))
+    a, b = api.DeviceBuffer(ext, prec), api.DeviceBuffer(ext, prec)
+    i = args.pass_index
+    desc = p.describe().strip().splitlines()[i]
+    for _ in range(args.warmup):
+        p.execute_pass(i, a.ptr, ext, b.ptr, ext, api.FORWARD)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    for _ in range(args.steps):
+        p.execute_pass(i, a.ptr, ext, b.ptr, ext, api.FORWARD)
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1) / args.steps
+    by = p.pass_bytes(i)
+    print(json.dumps({"metric": "one pass of one transform (GB/s, algorithmic bytes)", "job": name,
+                      "precision": "complex128" if prec == api.Z2Z else "complex64", "elements": elems,
+                      "pass": desc, "us": round(ms * 1e3, 1), "gbs": round(by / (ms * 1e-3) / 1e9, 1),
+                      "steps": args.steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -647,7 +684,10 @@ def main():
     ap.add_argument("--dims", type=str, default="512,512,512")
     ap.add_argument("--prec", type=str, default="z2z", choices=["c2c", "z2z", "any"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="fftnd", choices=["fftnd", "batch", "serve", "fft1d", "serve-passes"])
+    ap.add_argument("--workload", default="fftnd",
+                    choices=["fftnd", "batch", "serve", "fft1d", "serve-passes", "job-pass"])
+    ap.add_argument("--dims-nd", type=str, default="", help="job-pass: FFTND dims (else FFT1D of --n)")
+    ap.add_argument("--pass-index", type=int, default=0, help="job-pass: which pass of the plan")
     ap.add_argument("--n", type=int, default=1 << 27, help="fft1d length")
     ap.add_argument("--requests", type=int, default=2000, help="jobs in the serve workload")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve arrival rate (jobs/s)")
@@ -682,6 +722,9 @@ def main():
         return
     if args.workload == "serve-passes":
         bench_serve_passes(args, Hip())
+        return
+    if args.workload == "job-pass":
+        bench_job_pass(args, Hip())
         return
 
     from shafft_amd import api

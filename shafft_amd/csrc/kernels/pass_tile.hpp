@@ -60,11 +60,12 @@ __device__ __forceinline__ void gstore(cpx<R>* p, cpx<R> x, bool nt) {
 // ---------------------------------------------------------------------------
 
 // LDS -> LDS, butterflies jb = tl + i*T held in registers across a barrier.
-// Every slot computes a butterfly: a slot past the line's last butterfly
-// (jb >= nb) computes the last one again and only its writes are skipped,
-// so the reads, twiddles and DFT run without per-butterfly branches (a
-// guarded butterfly cost exec-mask branches and register moves at every
-// join).  Ns is wave-uniform (the twiddle branch is a scalar branch).
+// A slot runs when any lane of the wave has a butterfly there (a wave-uniform
+// branch); in a running slot a lane past the line's last butterfly
+// (jb >= nb) computes the last one again and only its writes are skipped, so
+// the reads, twiddles and DFT run without exec-mask branches (a guarded
+// butterfly cost those branches and register moves at every join).  Ns is
+// wave-uniform (the twiddle branch is a scalar branch).
 template <typename R, int RAD, int SIGN, int E = EMAX>
 __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdiv_t nsd,
                                           const cpx<R>* __restrict__ tw, int tl, int T) {
@@ -73,8 +74,11 @@ __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdi
   const FastDiv fd{nsd.d, nsd.m, nsd.s};
   cpx<R> v[MAXB][RAD];
   int idx[MAXB];
+  bool run[MAXB];
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
+    run[i] = __any(tl + i * T < nb) != 0;
+    if (!run[i]) continue;
     const int jb = min(tl + i * T, nb - 1);
 #pragma unroll
     for (int r = 0; r < RAD; ++r) v[i][r] = line[lp(jb + r * nb)];
@@ -91,7 +95,7 @@ __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdi
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
-    if (tl + i * T < nb) {
+    if (run[i] && tl + i * T < nb) {
 #pragma unroll
       for (int r = 0; r < RAD; ++r) line[lp(idx[i] + r * Ns)] = v[i][r];
     }
