@@ -11,7 +11,16 @@ namespace sa {
 // Register budget per thread per stage, in complex elements.
 constexpr int EMAX = 16;
 
-__device__ __forceinline__ int lp(int k) { return k + (k >> 4); }
+// Padded LDS index: one pad element every 16 (k -> k + k/16) keeps the
+// stride-R butterfly writes of the early stages off a single bank.  The
+// empty asm keeps the sum whole, so the address is one shift-and-add from
+// it (the compiler otherwise shifts both terms apart and adds three ways:
+// one more VALU instruction per LDS access).
+__device__ __forceinline__ int lp(int k) {
+  int v = k + (k >> 4);
+  asm("" : "+v"(v));
+  return v;
+}
 
 // Wave-uniform copy of a value that is the same in every lane (descriptor
 // fields read from memory).  Control flow that encloses a barrier must be
