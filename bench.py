@@ -479,6 +479,7 @@ def serve_measure(reqs, hip, slots=8, arena_gb=24, outstanding=48, full=True):
     sat_res = _serve_run(srv, reqs, inp, outs, paced=False)
     sat = _lat_stats(sat_res)
     st = srv.stats()
+    nb, nbj = srv.batch_stats()
     srv.close()
     gpu_busy_s = sum(r.gpu_ns for r in sat_res) * 1e-9
     # the plain path may overwrite its source (execute leaves the result in
@@ -504,6 +505,8 @@ def serve_measure(reqs, hip, slots=8, arena_gb=24, outstanding=48, full=True):
         "plain_per_call_gflops": round(gflop / plain, 2),
         "plain_per_call_req_per_s": round(len(reqs) / plain, 1),
         "speedup_vs_plain": round(plain / sat["span_s"], 3),
+        "micro_batched_jobs": nbj,
+        "micro_batches": nb,
         "plans_resident": st["plans"],
         "table_mb": round(st["table_bytes"] / 2**20, 2),
         "arena_peak_gb": round(st["arena_peak"] / 2**30, 3),
@@ -535,12 +538,13 @@ def serving_summary(hip, requests=2000, rate=2000.0):
         "workload": f"{requests} jobs, " + STREAM_DESC.format(rate=rate, lo=10, hi=26),
         **{k: full[k] for k in ("p50_ms", "p99_ms", "cold_p50_ms", "cold_p99_ms", "saturated_gflops",
                                 "saturated_req_per_s", "plain_per_call_gflops", "speedup_vs_plain",
-                                "gpu_busy_frac")},
+                                "gpu_busy_frac", "micro_batched_jobs", "micro_batches")},
         "target_speedup_vs_plain": 3.0,
         "substream_le_2p20": {
             "workload": f"{requests} jobs, " + STREAM_DESC.format(rate=rate, lo=10, hi=20),
             **{k: small[k] for k in ("p50_ms", "p99_ms", "saturated_req_per_s", "saturated_gflops",
-                                     "plain_per_call_req_per_s", "plain_per_call_gflops", "speedup_vs_plain")},
+                                     "plain_per_call_req_per_s", "plain_per_call_gflops", "speedup_vs_plain",
+                                     "micro_batched_jobs", "micro_batches")},
         },
     }
 

@@ -88,6 +88,8 @@ _SIGS = {
     "shafftAmdServerDrain": (ct.c_int, [ct.c_void_p]),
     "shafftAmdServerStats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_double)]),
     "shafftAmdServerResetStats": (ct.c_int, [ct.c_void_p]),
+    "shafftAmdServerSetBatching": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int]),
+    "shafftAmdServerBatchStats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
     "shafftAmdServerNow": (ct.c_int, [ct.POINTER(ct.c_int64)]),
     "shafftAmdServerDestroy": (ct.c_int, [ct.c_void_p]),
     # reference C API
@@ -644,6 +646,19 @@ class Server:
 
     def reset_stats(self):
         check(self.L.shafftAmdServerResetStats(self.h), "shafftAmdServerResetStats", self.L)
+
+    def set_batching(self, max_elems: int, max_jobs: int = 64):
+        """Micro-batch jobs of at most max_elems elements (0: off), up to
+        max_jobs per group (jobs admitted from now on)."""
+        check(self.L.shafftAmdServerSetBatching(self.h, int(max_elems), int(max_jobs)),
+              "shafftAmdServerSetBatching", self.L)
+
+    def batch_stats(self):
+        """(micro-batches formed, jobs they held) since the last reset."""
+        b, j = ct.c_int64(), ct.c_int64()
+        check(self.L.shafftAmdServerBatchStats(self.h, ct.byref(b), ct.byref(j)), "shafftAmdServerBatchStats",
+              self.L)
+        return b.value, j.value
 
     def close(self):
         if self.h.value:

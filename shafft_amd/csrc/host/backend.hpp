@@ -36,6 +36,7 @@ class NDBackend {
   void execute_preserve(const void* in, void* out, void* s1, void* s2, int dir,
                         hipStream_t stream) const;
   int64_t preserve_scratch_elems(int i) const { return map_pr_.scratch[i]; }
+  const PassProgram::SlotMap& preserve_map() const { return map_pr_; }
   int64_t table_bytes() const { return prog_.table_bytes(); }
 
  private:
@@ -64,6 +65,7 @@ class OneDBackend {
   void execute_in_scratch(const void* in, void* out, int dir, void* scratch,
                           size_t scratch_bytes) const;
   int64_t preserve_scratch_elems(int i) const { return map_pr_.scratch[i]; }
+  const PassProgram::SlotMap& preserve_map() const { return map_pr_; }
   int64_t table_bytes() const { return prog_.table_bytes(); }
   void set_stream(hipStream_t s) { stream_ = s; }
   hipStream_t stream() const { return stream_; }

@@ -30,6 +30,29 @@ static uint64_t wall_clock_hz() {
   return cached;
 }
 
+bool batch_retile(sa_pass_t& p, int elem_bytes, int block, int lds_budget) {
+  const int n = p.n;
+  const int T = (n + 15) / 16;
+  if (T > 512) return false;
+  int C = std::max(1, block / T);
+  int stride = n + (n - 1) / 16;
+  if (stride % 2 == 0) stride += 1;
+  while (C > 1 && (int64_t)C * stride * elem_bytes > lds_budget) --C;
+  if ((int64_t)C * stride * elem_bytes > lds_budget) return false;
+  if (C > p.batch) C = (int)std::max<int64_t>(1, p.batch);
+  p.T = T;
+  p.emax = 16;
+  p.C = C;
+  p.ldsstride = stride;
+  p.lds_pad = 1;  // the batch kernel runs padded rows
+  p.Tdiv = make_fastdiv((uint32_t)T);
+  p.Cdiv = make_fastdiv((uint32_t)C);
+  p.reg_first = p.reg_last = 0;
+  p.nt = 0;
+  p.logC = 0;
+  return true;
+}
+
 BatchPlan::BatchPlan(int count, const int* ndims, const int* dims_flat, int prec)
     : prec_(prec), prog_(prec) {
   if (prec != 0 && prec != 1) fail(ERR_INVALID_FFTTYPE, "invalid FFT type");
@@ -103,31 +126,17 @@ void BatchPlan::build() {
   sign[1].resize(np);
   block_ = 1;
   lds_ = 0;
-  const int budget = 159 * 1024;
+  const int budget = kBatchLdsBudget;
   for (const Tr& tr : tr_) {
     for (int j = 0; j < tr.np; ++j) {
       const size_t i = (size_t)(tr.first + j);
       const PassSpec& ps = prog_.pass(i);
       sa_pass_t p = ps.p;
       // Re-tile for the batch kernel (generic LDS path).
-      const int n = p.n;
-      const int T = (n + 15) / 16;
-      int C = std::max(1, sched_.block / T);
-      int stride = n + (n - 1) / 16;
-      if (stride % 2 == 0) stride += 1;
-      while (C > 1 && (int64_t)C * stride * esz > budget) --C;
-      if (C > p.batch) C = (int)std::max<int64_t>(1, p.batch);
-      p.T = T;
-      p.emax = 16;
-      p.C = C;
-      p.ldsstride = stride;
-      p.Tdiv = make_fastdiv((uint32_t)T);
-      p.Cdiv = make_fastdiv((uint32_t)C);
-      p.reg_first = p.reg_last = 0;
-      p.nt = 0;
-      p.logC = 0;
-      block_ = std::max(block_, (C * T + 63) / 64 * 64);  // whole waves
-      lds_ = std::max(lds_, (size_t)C * stride * esz);
+      if (!batch_retile(p, esz, sched_.block, budget)) fail(ERR_INTERNAL, "batched pass does not fit LDS");
+      const int C = p.C;
+      block_ = std::max(block_, (C * p.T + 63) / 64 * 64);  // whole waves
+      lds_ = std::max(lds_, (size_t)C * p.ldsstride * esz);
       tiles[i] = (uint32_t)((p.batch + C - 1) / C);
       if (j > 0) dep[i] = (int32_t)(i - 1);
       // Buffers: pass j writes `out` when (np-1-j) is even, else `work`.

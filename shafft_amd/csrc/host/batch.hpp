@@ -25,6 +25,16 @@ struct BatchSchedule {
   bool levels = true; // one launch per pass level instead of dependency waits
 };
 
+// Re-tile pass descriptor `p` for the persistent batch kernel (generic LDS
+// path, padded rows, 16 elements per thread, no register stages): tiles of
+// C lines of T = ceil(n / 16) threads within `block` threads and `lds_budget`
+// bytes.  Returns false when one line does not fit.
+bool batch_retile(sa_pass_t& p, int elem_bytes, int block, int lds_budget);
+
+// LDS bytes the batch kernel may use per workgroup (159 KiB: its own queue
+// slot sits in static LDS).
+constexpr int kBatchLdsBudget = 159 * 1024;
+
 class BatchPlan {
  public:
   BatchPlan(int count, const int* ndims, const int* dims_flat, int prec);

@@ -577,6 +577,24 @@ int shafftAmdServerStats(shafft_amd_server* srv, int64_t* counts, double* values
   });
 }
 
+int shafftAmdServerSetBatching(shafft_amd_server* srv, int64_t max_elems, int max_jobs) {
+  return guarded([&] {
+    if (!srv) fail(ERR_NULLPTR, "null server");
+    srv->be->set_batching(max_elems, max_jobs);
+    return 0;
+  });
+}
+
+int shafftAmdServerBatchStats(shafft_amd_server* srv, int64_t* batches, int64_t* batched_jobs) {
+  return guarded([&] {
+    if (!srv || !batches || !batched_jobs) fail(ERR_NULLPTR, "null argument");
+    const ServeStats s = srv->be->stats();
+    *batches = s.batches;
+    *batched_jobs = s.batched_jobs;
+    return 0;
+  });
+}
+
 int shafftAmdServerResetStats(shafft_amd_server* srv) {
   return guarded([&] {
     if (!srv) fail(ERR_NULLPTR, "null server");

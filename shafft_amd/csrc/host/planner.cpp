@@ -330,6 +330,153 @@ static int env_int(const char* name, int dflt) {
 
 static bool pow2_le16(int r) { return r == 2 || r == 4 || r == 8 || r == 16; }
 
+// LDS bank cycles of one tile of the generic pass, summed over every LDS
+// access instruction of every wave, for padded (k -> k + k/16) or unpadded
+// rows of `stride` elements.  Model: `banks` banks of 4 bytes; a wave's
+// access of `w` dwords per lane is served in groups of banks/w lanes, each
+// group taking as many cycles as the most distinct dwords that fall on one
+// bank.  Follows the kernel's lane mappings (pass_tile.hpp): the load's and
+// store's line slots, the butterfly identity of the stages, clamped
+// butterflies in running slots, one butterfly per thread for radices > 16,
+// and the register-resident first and last stages.
+static int64_t lds_bank_cycles(const sa_pass_t& p, int elem_bytes, bool pad, int stride, int banks) {
+  const int n = p.n, T = p.T, C = p.C, E = p.emax;
+  const int nthr = C * T;
+  const int w = elem_bytes / 4;
+  const int group = std::max(1, banks / w);
+  auto lpk = [pad](int k) { return pad ? k + (k >> 4) : k; };
+  auto line_of = [&](int t, bool kfast) { return kfast ? t / T : t % C; };
+  auto elem_of = [&](int t, bool kfast) { return kfast ? t % T : t / C; };
+  int64_t total = 0;
+  std::vector<int> cnt(banks);
+  std::vector<int64_t> addr(64);
+  std::vector<char> on(64);
+  // One instruction of one wave: lanes with on[l] access element addr[l].
+  auto issue = [&](int lanes) {
+    for (int g0 = 0; g0 < lanes; g0 += group) {
+      std::fill(cnt.begin(), cnt.end(), 0);
+      int worst = 0;
+      for (int l = g0; l < std::min(lanes, g0 + group); ++l) {
+        if (!on[l]) continue;
+        bool dup = false;  // same element as an earlier lane of the group: one access
+        for (int m = g0; m < l && !dup; ++m) dup = on[m] && addr[m] == addr[l];
+        if (dup) continue;
+        for (int d = 0; d < w; ++d) {
+          const int b = (int)((addr[l] * w + d) % banks);
+          worst = std::max(worst, ++cnt[b]);
+        }
+      }
+      total += worst;
+    }
+  };
+  const bool id_kfast = p.reg_first ? p.ld_kfast != 0 : p.st_kfast != 0;
+  for (int w0 = 0; w0 < nthr; w0 += 64) {
+    const int lanes = std::min(64, nthr - w0);
+    // load into LDS
+    if (!p.reg_first) {
+      for (int i = 0; i < E; ++i) {
+        for (int l = 0; l < lanes; ++l) {
+          const int t = w0 + l, k = elem_of(t, p.ld_kfast != 0) + i * T;
+          on[l] = k < n;
+          addr[l] = (int64_t)line_of(t, p.ld_kfast != 0) * stride + lpk(k);
+        }
+        issue(lanes);
+      }
+    }
+    const int s0 = p.reg_first ? 1 : 0;
+    const int s1 = p.reg_last ? p.nstages - 1 : p.nstages;
+    if (p.reg_first) {
+      const int R = p.radix[0];
+      for (int i = 0; i < 16 / R; ++i)
+        for (int r = 0; r < R; ++r) {
+          for (int l = 0; l < lanes; ++l) {
+            const int t = w0 + l;
+            on[l] = 1;
+            addr[l] = (int64_t)line_of(t, id_kfast) * stride + lpk((elem_of(t, id_kfast) + i * T) * R + r);
+          }
+          issue(lanes);
+        }
+    }
+    for (int s = s0; s < s1; ++s) {
+      const int R = p.radix[s], Ns = p.ns[s], nb = n / R;
+      if (R > 16) {
+        for (int phase = 0; phase < 2; ++phase)
+          for (int r = 0; r < R; ++r) {
+            for (int l = 0; l < lanes; ++l) {
+              const int t = w0 + l;
+              on[l] = t < C * nb;
+              const int c = t / nb, jb = t % nb;
+              const int q = jb / Ns, rem = jb - q * Ns;
+              const int k = phase == 0 ? jb + r * nb : q * Ns * R + rem + r * Ns;
+              addr[l] = (int64_t)c * stride + lpk(k);
+            }
+            issue(lanes);
+          }
+        continue;
+      }
+      const int maxb = (E + R - 1) / R;
+      for (int i = 0; i < maxb; ++i) {
+        bool run = false;
+        for (int l = 0; l < lanes && !run; ++l) run = elem_of(w0 + l, id_kfast) + i * T < nb;
+        if (!run) continue;
+        for (int phase = 0; phase < 2; ++phase)
+          for (int r = 0; r < R; ++r) {
+            for (int l = 0; l < lanes; ++l) {
+              const int t = w0 + l, tl = elem_of(t, id_kfast);
+              const int jb = std::min(tl + i * T, nb - 1);
+              const int q = jb / Ns, rem = jb - q * Ns;
+              on[l] = phase == 0 || tl + i * T < nb;
+              const int k = phase == 0 ? jb + r * nb : q * Ns * R + rem + r * Ns;
+              addr[l] = (int64_t)line_of(t, id_kfast) * stride + lpk(k);
+            }
+            issue(lanes);
+          }
+      }
+    }
+    if (p.reg_last) {
+      const int R = p.radix[p.nstages - 1], nb = n / R;
+      for (int i = 0; i < 16 / R; ++i)
+        for (int r = 0; r < R; ++r) {
+          for (int l = 0; l < lanes; ++l) {
+            const int t = w0 + l;
+            on[l] = 1;
+            addr[l] = (int64_t)line_of(t, id_kfast) * stride + lpk(elem_of(t, id_kfast) + i * T + r * nb);
+          }
+          issue(lanes);
+        }
+    } else {
+      for (int i = 0; i < E; ++i) {
+        for (int l = 0; l < lanes; ++l) {
+          const int t = w0 + l;
+          on[l] = 1;
+          addr[l] = (int64_t)line_of(t, p.st_kfast != 0) * stride +
+                    lpk(std::min(elem_of(t, p.st_kfast != 0) + i * T, n - 1));
+        }
+        issue(lanes);
+      }
+    }
+  }
+  return total;
+}
+
+// Padded or unpadded LDS rows for a generic pass (p.C, p.T, lane mappings
+// and register-stage flags already set).  Unpadded rows save three VALU
+// instructions per LDS access (the generic pass is VALU-issue bound) and
+// are taken when the bank model gives them at most `slack` times the padded
+// rows' cycles, on both a 64-bank and a 32-bank LDS.
+//   SHAFFT_AMD_LDS_PAD: 1 always padded, 0 always unpadded, -1 (default) model.
+static bool choose_lds_pad(const sa_pass_t& p, int elem_bytes, int padded_stride, int plain_stride) {
+  const int knob = env_int("SHAFFT_AMD_LDS_PAD", -1);
+  if (knob >= 0) return knob != 0;
+  const double slack = env_int("SHAFFT_AMD_LDS_SLACK_PCT", 110) / 100.0;
+  for (int banks : {64, 32}) {
+    const int64_t cp = lds_bank_cycles(p, elem_bytes, true, padded_stride, banks);
+    const int64_t cu = lds_bank_cycles(p, elem_bytes, false, plain_stride, banks);
+    if ((double)cu > slack * (double)cp) return true;
+  }
+  return false;
+}
+
 // Fill launch shape: threads per line, lines per workgroup, lane mappings,
 // LDS stride and the register-resident stage flags.
 static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
@@ -394,6 +541,17 @@ static void configure_launch(sa_pass_t& p, int elem_bytes, bool bluestein) {
                     p.ld_kfast == p.st_kfast;
   p.reg_first = regs ? 1 : 0;
   p.reg_last = regs ? 1 : 0;
+  // Unpadded LDS rows (odd stride, so line-fast lanes still spread) when
+  // the bank model allows; Bluestein passes keep the padded rows.
+  p.lds_pad = 1;
+  if (!bluestein) {
+    const int plain = n % 2 ? n : n + 1;
+    if (!choose_lds_pad(p, elem_bytes, stride, plain)) {
+      p.lds_pad = 0;
+      stride = plain;
+      p.ldsstride = stride;
+    }
+  }
   p.nt = env_int("SHAFFT_AMD_NT", 1) ? 1 : 0;
   // XCD-contiguous tile order (each XCD walks its own range of adjacent
   // tiles) for passes with a line-fast side of row segments up to 1 KiB:
@@ -492,6 +650,7 @@ static bool configure_fixed(sa_pass_t& p, int prec, int64_t& grid) {
   p.Tdiv = make_fastdiv((uint32_t)T);
   p.Cdiv = make_fastdiv((uint32_t)p.C);
   p.reg_first = p.reg_last = 0;
+  p.lds_pad = 1;  // the fixed kernel's rows are padded
   return true;
 }
 
@@ -938,6 +1097,7 @@ bool PassProgram::configure_blue(PassSpec& ps, int kind) {
   p.Cdiv = make_fastdiv((uint32_t)p.C);
   p.reg_first = p.reg_last = 0;
   p.xcd_order = 0;
+  p.lds_pad = 1;  // the Bluestein kernels' rows are padded
   ps.blue = kind;
   return true;
 }
@@ -1130,7 +1290,7 @@ std::string PassProgram::describe() const {
     os << " iks=" << p.iks << " oks=" << p.oks << " C=" << p.C << " T=" << p.T
        << " ld=" << (p.ld_kfast ? "k" : "line") << " st=" << (p.st_kfast ? "k" : "line")
        << (p.blu_h || passes_[i].blu_t[0] ? " bluestein" : "")
-       << (p.ptw_level >= 0 ? " twiddle" : "");
+       << (p.ptw_level >= 0 ? " twiddle" : "") << (p.lds_pad ? "" : " unpadded");
     if (p.ld_level >= 0 || p.ld_limit < p.n || p.pre || passes_[i].pre_t[0])
       os << " ld_hook(kmul=" << p.ld_kmul << ",lvl=" << p.ld_level << "/" << p.ld_level2
          << ",lim=" << p.ld_limit << ")";

@@ -16,6 +16,16 @@
 //    admitted when a stream slot is free and its scratch footprint fits in
 //    the arena; jobs queue in arrival order and a later job that fits may
 //    pass an earlier one that does not, at most `max_bypass` times.
+//  * Small jobs (up to mb_max_elems elements) are micro-batched: the jobs
+//    admitted in one scheduling round are grouped per precision (up to
+//    mb_max_jobs a group) and a group runs as one persistent batch-kernel
+//    launch per pass level on one slot (kernels/fft_batch.hip), its pass
+//    descriptors and work list copied from pinned host staging in one
+//    transfer.  A job's result then depends only on its own data (the batch
+//    kernel's tiles never mix jobs), so it is the same alone and in any mix;
+//    it equals the plain call's result within the precision's tolerance (the
+//    batch kernel runs the generic LDS path where a plain call may use the
+//    fixed-length kernel).  Larger jobs run their own pass launches on a slot.
 //  * Completion is observed by polling per-slot events; latency is measured
 //    from the caller-given arrival time to the observed completion.
 #pragma once
@@ -63,6 +73,7 @@ struct ServeStats {
   double p50_ms = 0, p99_ms = 0, mean_ms = 0, max_ms = 0;
   double span_s = 0;   // first arrival -> last completion among completed jobs
   double gflop = 0;    // 5 N log2 N summed over completed jobs
+  int64_t batches = 0, batched_jobs = 0;  // micro-batches formed, jobs they held
 };
 
 // First-fit allocator over one device allocation (offsets 256-byte aligned).
@@ -91,6 +102,8 @@ class Server {
     int64_t arena_bytes = int64_t(16) << 30;
     int planners = 2;
     int max_bypass = 64;
+    int64_t mb_max_elems = int64_t(1) << 20;  // micro-batch jobs up to this size (0: off)
+    int mb_max_jobs = 64;                     // jobs per micro-batch
   };
   explicit Server(const Config& cfg);
   ~Server();
@@ -108,6 +121,8 @@ class Server {
   // 0 finished, 1 queued or running; unknown ids are an error.
   int query(int64_t id) const;
   void drain();  // block until nothing is queued or running
+  // Micro-batching limits for jobs admitted from now on (max_elems 0: off).
+  void set_batching(int64_t max_elems, int max_jobs);
   ServeStats stats() const;
   void reset_stats();
   static int64_t now_ns();
@@ -122,8 +137,19 @@ class Server {
     int64_t scratch[2] = {0, 0};  // bytes
     int status = 0;
     std::string msg;
+    // Batch form of the input-preserving pass program: descriptors re-tiled
+    // for the batch kernel (buffers unset), tiles per pass, launch shape.
+    bool mb_ok = false;
+    const PassProgram* prog = nullptr;
+    PassProgram::SlotMap map;
+    std::vector<sa_pass_t> mb_pass;
+    std::vector<uint32_t> mb_tiles;
+    int64_t mb_items = 0;
+    int mb_block = 0;
+    size_t mb_lds = 0;
   };
   struct Job {
+    int64_t id = 0;
     ServeJob d;
     const Plan* plan = nullptr;
     const void* in = nullptr;
@@ -136,16 +162,28 @@ class Server {
     int slot = -1;
     int64_t off[2] = {-1, -1}, bytes[2] = {0, 0};
   };
+  static constexpr int64_t kSlotFree = -1, kSlotBatch = -2;  // Slot::job markers
   struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, end = nullptr;
-    int64_t job = -1;
+    int64_t job = kSlotFree;  // job id, kSlotFree, or kSlotBatch (micro-batch)
     bool launching = false;
+    // micro-batch on this slot: its jobs, passes and work items so far
+    std::vector<int64_t> mb_jobs;
+    int mb_prec = 0;
+    int64_t mb_np = 0, mb_items = 0;
+    void* mb_host = nullptr;  // pinned staging of descriptors and work list
+    void* mb_dev = nullptr;   // device copy the batch kernel reads
   };
+  static constexpr size_t kMbStageBytes = size_t(4) << 20;  // per slot
 
   void scheduler();
   void planner();
   int launch(Job& j, const Plan& p, Slot& s, std::string& msg);  // lock released
+  int launch_mb(Slot& s, const std::vector<Job*>& js, std::string& msg);  // lock released
+  bool mb_eligible(const Job& j, const Plan& p) const;
+  static size_t mb_stage_bytes(int64_t np, int64_t items);
+  void build_mb(Plan& p) const;  // batch form of a READY plan (planner thread)
   void finish(Job& j, int status, const std::string& msg);       // lock held
   void release(Job& j);                                          // lock held
   void destroy_slots();
@@ -164,10 +202,12 @@ class Server {
   int device_ = 0;
   bool stop_ = false;
   bool dirty_ = false;  // queue, plans or slots changed since the last admission scan
+  std::map<std::pair<int, std::pair<int, size_t>>, int> occ_cache_;  // scheduler thread
   std::vector<std::thread> planners_;
   std::thread sched_;
   // statistics (lock held)
   int64_t n_submitted_ = 0, n_completed_ = 0, n_failed_ = 0, n_builds_ = 0;
+  int64_t n_batches_ = 0, n_batched_jobs_ = 0;
   std::vector<int64_t> lat_ns_;
   int64_t first_arrival_ = INT64_MAX, last_done_ = 0;
   double gflop_ = 0;

@@ -43,27 +43,29 @@ __device__ __forceinline__ uint32_t tile_of_block(const sa_pass_t& a) {
 // over 3 stages, load and store, by SQ counters).
 // E: elements per thread (8 only for the in-LDS Bluestein rows of the
 // three-pass long Bluestein: twice the threads on a pass of few, long rows).
-template <int SIGN, int MODE, bool TWL, int E = EMAX>
+// PAD: LDS rows padded (k -> k + k/16); unpadded for passes whose accesses
+// spread over the banks without it (the planner's choice, a.lds_pad).
+template <int SIGN, int MODE, bool TWL, int E = EMAX, bool PAD = true>
 __global__ __launch_bounds__(512) void fft_lds_pass_f(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<float, SIGN, MODE, 0, TWL, E>(a, tile_of_block(a), reinterpret_cast<cpx<float>*>(lds_raw));
+  pass_tile<float, SIGN, MODE, 0, TWL, E, PAD>(a, tile_of_block(a), reinterpret_cast<cpx<float>*>(lds_raw));
 }
 
-template <int SIGN, int MODE, bool TWL, int E = EMAX>
+template <int SIGN, int MODE, bool TWL, int E = EMAX, bool PAD = true>
 __global__ __launch_bounds__(512) void fft_lds_pass_d(const sa_pass_t a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_tile<double, SIGN, MODE, 0, TWL, E>(a, tile_of_block(a), reinterpret_cast<cpx<double>*>(lds_raw));
+  pass_tile<double, SIGN, MODE, 0, TWL, E, PAD>(a, tile_of_block(a), reinterpret_cast<cpx<double>*>(lds_raw));
 }
 
-template <typename R, int SIGN, int MODE, bool TWL, int E = EMAX>
+template <typename R, int SIGN, int MODE, bool TWL, int E = EMAX, bool PAD = true>
 struct PassKernel;
-template <int SIGN, int MODE, bool TWL, int E>
-struct PassKernel<float, SIGN, MODE, TWL, E> {
-  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_f<SIGN, MODE, TWL, E>); }
+template <int SIGN, int MODE, bool TWL, int E, bool PAD>
+struct PassKernel<float, SIGN, MODE, TWL, E, PAD> {
+  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_f<SIGN, MODE, TWL, E, PAD>); }
 };
-template <int SIGN, int MODE, bool TWL, int E>
-struct PassKernel<double, SIGN, MODE, TWL, E> {
-  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_d<SIGN, MODE, TWL, E>); }
+template <int SIGN, int MODE, bool TWL, int E, bool PAD>
+struct PassKernel<double, SIGN, MODE, TWL, E, PAD> {
+  static const void* fn() { return reinterpret_cast<const void*>(&fft_lds_pass_d<SIGN, MODE, TWL, E, PAD>); }
 };
 
 template <typename R>
@@ -78,27 +80,30 @@ __global__ __launch_bounds__(256) void scale_kernel(cpx<R>* __restrict__ x, R f,
 }
 
 // TWL: the stage twiddles are staged in LDS behind the tile (a.tw_lds > 0).
-template <typename R, int SIGN, int MODE, bool TWL, int E = EMAX>
+template <typename R, int SIGN, int MODE, bool TWL, int E = EMAX, bool PAD = true>
 static hipError_t launch_t(const sa_pass_t* p, hipStream_t st) {
   const uint32_t nthr = (uint32_t)(p->C * p->T);
   const uint64_t nblk = ((uint64_t)p->batch + p->C - 1) / p->C;
   const size_t lds = ((size_t)p->C * p->ldsstride + (TWL ? (size_t)p->tw_lds : 0)) * sizeof(cpx<R>);
   static const bool attr = [] {  // once per instantiation, thread-safe
-    (void)hipFuncSetAttribute(PassKernel<R, SIGN, MODE, TWL, E>::fn(),
+    (void)hipFuncSetAttribute(PassKernel<R, SIGN, MODE, TWL, E, PAD>::fn(),
                               hipFuncAttributeMaxDynamicSharedMemorySize, sa_max_lds_bytes());
     (void)hipGetLastError();
     return true;
   }();
   (void)attr;
   if constexpr (sizeof(R) == 4)
-    hipLaunchKernelGGL((fft_lds_pass_f<SIGN, MODE, TWL, E>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
+    hipLaunchKernelGGL((fft_lds_pass_f<SIGN, MODE, TWL, E, PAD>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
   else
-    hipLaunchKernelGGL((fft_lds_pass_d<SIGN, MODE, TWL, E>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
+    hipLaunchKernelGGL((fft_lds_pass_d<SIGN, MODE, TWL, E, PAD>), dim3((uint32_t)nblk), dim3(nthr), lds, st, *p);
   return hipGetLastError();
 }
 
 template <typename R, int SIGN, int MODE>
 static hipError_t launch_tw(const sa_pass_t* p, hipStream_t st) {
+  if (!p->lds_pad)
+    return p->tw_lds > 0 ? launch_t<R, SIGN, MODE, true, EMAX, false>(p, st)
+                         : launch_t<R, SIGN, MODE, false, EMAX, false>(p, st);
   return p->tw_lds > 0 ? launch_t<R, SIGN, MODE, true>(p, st) : launch_t<R, SIGN, MODE, false>(p, st);
 }
 
@@ -128,6 +133,13 @@ static const void* pass_kernel(const sa_pass_t* p, bool twl) {
       if (p->emax == 8)
         return twl ? PassKernel<R, -1, MODE_BLUE, true, 8>::fn() : PassKernel<R, -1, MODE_BLUE, false, 8>::fn();
     return twl ? PassKernel<R, -1, MODE_BLUE, true>::fn() : PassKernel<R, -1, MODE_BLUE, false>::fn();
+  }
+  if (!p->lds_pad) {
+    if (p->reg_first && p->reg_last)
+      return twl ? PassKernel<R, -1, MODE_REGS, true, EMAX, false>::fn()
+                 : PassKernel<R, -1, MODE_REGS, false, EMAX, false>::fn();
+    return twl ? PassKernel<R, -1, MODE_LDS, true, EMAX, false>::fn()
+               : PassKernel<R, -1, MODE_LDS, false, EMAX, false>::fn();
   }
   if (p->reg_first && p->reg_last)
     return twl ? PassKernel<R, -1, MODE_REGS, true>::fn() : PassKernel<R, -1, MODE_REGS, false>::fn();

@@ -58,6 +58,7 @@ typedef struct {
   int32_t logC;      /* log2(C) for the fixed-length kernel (C a power of 2) */
   int32_t tw_lds;    /* fixed kernel: stage-twiddle elements staged in LDS (0: global) */
   int32_t xcd_order; /* fixed kernel: 1 = each XCD walks its own contiguous tile range */
+  int32_t lds_pad;   /* generic kernel: 1 = LDS rows padded k -> k + k/16, 0 = unpadded */
   sa_fastdiv_t Tdiv, Cdiv, n1div, n2div;
 
   /* Optional post-twiddle on the store: out *= W_N^{+-(digit * k) mod N}

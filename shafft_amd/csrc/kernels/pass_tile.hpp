@@ -11,16 +11,26 @@ namespace sa {
 // Register budget per thread per stage, in complex elements.
 constexpr int EMAX = 16;
 
-// Padded LDS index: one pad element every 16 (k -> k + k/16) keeps the
-// stride-R butterfly writes of the early stages off a single bank.  The
-// empty asm keeps the sum whole, so the address is one shift-and-add from
-// it (the compiler otherwise shifts both terms apart and adds three ways:
-// one more VALU instruction per LDS access).
-__device__ __forceinline__ int lp(int k) {
-  int v = k + (k >> 4);
-  asm("" : "+v"(v));
-  return v;
+// LDS index of element k of a line.  Padded rows (PAD): one pad element
+// every 16 (k -> k + k/16) keeps the stride-R butterfly writes of the early
+// stages off a single bank; the empty asm keeps the sum whole, so the
+// address is one shift-and-add from it (the compiler otherwise shifts both
+// terms apart and adds three ways: one more VALU instruction per access).
+// Unpadded rows (!PAD) are for passes whose stage accesses already spread
+// over the banks (odd radices, line-fast lanes): an access is then the
+// line's base plus a wave-uniform offset, one add instead of four VALU
+// instructions, and the register stages' accesses are immediate offsets.
+template <bool PAD = true>
+__device__ __forceinline__ int lpx(int k) {
+  if constexpr (PAD) {
+    int v = k + (k >> 4);
+    asm("" : "+v"(v));
+    return v;
+  } else {
+    return k;
+  }
 }
+__device__ __forceinline__ int lp(int k) { return lpx<true>(k); }
 
 // Wave-uniform copy of a value that is the same in every lane (descriptor
 // fields read from memory).  Control flow that encloses a barrier must be
@@ -75,7 +85,7 @@ __device__ __forceinline__ void gstore(cpx<R>* p, cpx<R> x, bool nt) {
 // the reads, twiddles and DFT run without exec-mask branches (a guarded
 // butterfly cost those branches and register moves at every join).  Ns is
 // wave-uniform (the twiddle branch is a scalar branch).
-template <typename R, int RAD, int SIGN, int E = EMAX>
+template <typename R, int RAD, int SIGN, int E = EMAX, bool PAD = true>
 __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdiv_t nsd,
                                           const cpx<R>* __restrict__ tw, int tl, int T) {
   constexpr int MAXB = (E + RAD - 1) / RAD;
@@ -90,7 +100,7 @@ __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdi
     if (!run[i]) continue;
     const int jb = min(tl + i * T, nb - 1);
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) v[i][r] = line[lp(jb + r * nb)];
+    for (int r = 0; r < RAD; ++r) v[i][r] = line[lpx<PAD>(jb + r * nb)];
     const int q = (int)fdiv((uint32_t)jb, fd);
     const int rem = jb - q * Ns;
     if (Ns > 1) {
@@ -106,7 +116,7 @@ __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdi
   for (int i = 0; i < MAXB; ++i) {
     if (run[i] && tl + i * T < nb) {
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) line[lp(idx[i] + r * Ns)] = v[i][r];
+      for (int r = 0; r < RAD; ++r) line[lpx<PAD>(idx[i] + r * Ns)] = v[i][r];
     }
   }
   __syncthreads();
@@ -116,7 +126,7 @@ __device__ __forceinline__ void stage_lds(cpx<R>* line, int n, int Ns, sa_fastdi
 // (n/16), so the tile's C*n/RAD butterflies are numbered densely over all
 // C*T threads (butterfly g = t: line g / (n/RAD)), one round, rather than
 // leaving a 1 - 16/RAD share of every line's threads idle.
-template <typename R, int RAD, int SIGN>
+template <typename R, int RAD, int SIGN, bool PAD = true>
 __device__ __forceinline__ void stage_lds_wide(cpx<R>* lds, int stride, int C, int nthr, int n,
                                                int Ns, sa_fastdiv_t nsd, sa_fastdiv_t nbd,
                                                const cpx<R>* __restrict__ tw, int t) {
@@ -131,7 +141,7 @@ __device__ __forceinline__ void stage_lds_wide(cpx<R>* lds, int stride, int C, i
   cpx<R> v[RAD];
   if (ok) {
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) v[r] = line[lp(jb + r * nb)];
+    for (int r = 0; r < RAD; ++r) v[r] = line[lpx<PAD>(jb + r * nb)];
     if (Ns > 1) {
       const int q = (int)fdiv((uint32_t)jb, fd);
       const int rem = jb - q * Ns;
@@ -147,7 +157,7 @@ __device__ __forceinline__ void stage_lds_wide(cpx<R>* lds, int stride, int C, i
     const int rem = jb - q * Ns;
     const int idx = q * Ns * RAD + rem;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) line[lp(idx + r * Ns)] = v[r];
+    for (int r = 0; r < RAD; ++r) line[lpx<PAD>(idx + r * Ns)] = v[r];
   }
   __syncthreads();
 }
@@ -157,7 +167,7 @@ __device__ __forceinline__ void stage_lds_wide(cpx<R>* lds, int stride, int C, i
 namespace sa {
 
 // Stage 0 from registers (v[m] = element tl + T*m, n == EMAX*T) into LDS.
-template <typename R, int RAD, int SIGN>
+template <typename R, int RAD, int SIGN, bool PAD = true>
 __device__ __forceinline__ void stage_first_regs(const cpx<R> (&v)[EMAX], cpx<R>* line, int tl,
                                                  int T, bool act) {
   constexpr int MAXB = EMAX / RAD;
@@ -169,14 +179,14 @@ __device__ __forceinline__ void stage_first_regs(const cpx<R> (&v)[EMAX], cpx<R>
     dft<RAD, SIGN>(u);
     const int idx = (tl + i * T) * RAD;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) line[lp(idx + r)] = u[r];
+    for (int r = 0; r < RAD; ++r) line[lpx<PAD>(idx + r)] = u[r];
   }
   __syncthreads();
 }
 
 // Last stage (Ns = n/RAD) from LDS into registers: v[i + r*MAXB] receives
 // output element jb + r*Ns = tl + T*(i + r*MAXB).
-template <typename R, int RAD, int SIGN>
+template <typename R, int RAD, int SIGN, bool PAD = true>
 __device__ __forceinline__ void stage_last_regs(cpx<R> (&v)[EMAX], const cpx<R>* line, int n,
                                                 const cpx<R>* __restrict__ tw, int tl, int T,
                                                 bool act) {
@@ -187,7 +197,7 @@ __device__ __forceinline__ void stage_last_regs(cpx<R> (&v)[EMAX], const cpx<R>*
     const int jb = tl + i * T;
     cpx<R> u[RAD];
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) u[r] = line[lp(jb + r * nb)];
+    for (int r = 0; r < RAD; ++r) u[r] = line[lpx<PAD>(jb + r * nb)];
     const cpx<R>* w = tw + jb * (RAD - 1);
 #pragma unroll
     for (int r = 1; r < RAD; ++r) u[r] = cmul_dir<SIGN>(u[r], w[r - 1]);
@@ -209,26 +219,26 @@ __device__ __forceinline__ int stage_lane(uint32_t tl) {
 }
 
 // One Stockham stage of radix `rad` (wave-uniform) on this thread's line.
-template <typename R, int SIGN, int E = EMAX>
+template <typename R, int SIGN, int E = EMAX, bool PAD = true>
 __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t nsd, sa_fastdiv_t nbd,
                                           const cpx<R>* tws, cpx<R>* line, int tl, int T,
                                           cpx<R>* lds, int stride, int C, int t) {
   switch (rad) {
-    case 2: stage_lds<R, 2, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 3: stage_lds<R, 3, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 4: stage_lds<R, 4, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 5: stage_lds<R, 5, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 6: stage_lds<R, 6, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 7: stage_lds<R, 7, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 8: stage_lds<R, 8, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 9: stage_lds<R, 9, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 10: stage_lds<R, 10, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 12: stage_lds<R, 12, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 16: stage_lds<R, 16, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 14: stage_lds<R, 14, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
-    case 15: stage_lds<R, 15, SIGN, E>(line, n, Ns, nsd, tws, tl, T); break;
+    case 2: stage_lds<R, 2, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 3: stage_lds<R, 3, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 4: stage_lds<R, 4, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 5: stage_lds<R, 5, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 6: stage_lds<R, 6, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 7: stage_lds<R, 7, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 8: stage_lds<R, 8, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 9: stage_lds<R, 9, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 10: stage_lds<R, 10, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 12: stage_lds<R, 12, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 16: stage_lds<R, 16, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 14: stage_lds<R, 14, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
+    case 15: stage_lds<R, 15, SIGN, E, PAD>(line, n, Ns, nsd, tws, tl, T); break;
 #define SA_WIDE(RR) \
-    case RR: stage_lds_wide<R, RR, SIGN>(lds, stride, C, C * T, n, Ns, nsd, nbd, tws, t); break;
+    case RR: stage_lds_wide<R, RR, SIGN, PAD>(lds, stride, C, C * T, n, Ns, nsd, nbd, tws, t); break;
     SA_WIDE(18) SA_WIDE(20) SA_WIDE(21)
 #undef SA_WIDE
     // Radices 24, 25, 28 for complex double only: in the complex-float
@@ -236,7 +246,7 @@ __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t n
 #define SA_WIDE_D(RR)                                                                    \
     case RR:                                                                             \
       if constexpr (sizeof(R) == 8)                                                      \
-        stage_lds_wide<R, RR, SIGN>(lds, stride, C, C * T, n, Ns, nsd, nbd, tws, t);     \
+        stage_lds_wide<R, RR, SIGN, PAD>(lds, stride, C, C * T, n, Ns, nsd, nbd, tws, t);     \
       break;
     SA_WIDE_D(24) SA_WIDE_D(25) SA_WIDE_D(28)
 #undef SA_WIDE_D
@@ -247,12 +257,12 @@ __device__ __forceinline__ void stage_run(int rad, int n, int Ns, sa_fastdiv_t n
 // MX != 0 (complex float only): radix-8/16 stages on the matrix cores in
 // split precision (mfma_stage.hpp); `lds` is the tile's LDS base.
 // twb: the stage twiddle table (global, or staged in LDS by the caller).
-template <typename R, int SIGN, int MX = 0, int E = EMAX>
+template <typename R, int SIGN, int MX = 0, int E = EMAX, bool PAD = true>
 __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>* line, int tl,
                                                cpx<R>* lds, const cpx<R>* twb) {
   const cpx<R>* tws = twb + a.twoff[s];
   if constexpr (MX != 0) {
-    static_assert(sizeof(R) == 4, "split-precision stages are for complex float");
+    static_assert(sizeof(R) == 4 && PAD, "split-precision stages: complex float, padded rows");
     const int rad = uni(a.radix[s]);
     if (rad == 16) {
       stage_mfma<16, SIGN, MX>(lds, a.C, a.ldsstride, a.n, a.ns[s], a.nsdiv[s], tws);
@@ -263,33 +273,33 @@ __device__ __forceinline__ void stage_dispatch(const sa_pass_t& a, int s, cpx<R>
       return;
     }
   }
-  stage_run<R, SIGN, E>(uni(a.radix[s]), a.n, uni(a.ns[s]), a.nsdiv[s], a.nbdiv[s], tws, line, tl, a.T,
+  stage_run<R, SIGN, E, PAD>(uni(a.radix[s]), a.n, uni(a.ns[s]), a.nsdiv[s], a.nbdiv[s], tws, line, tl, a.T,
                      lds, a.ldsstride, a.C, stage_lane(threadIdx.x));
 }
 
-template <typename R, int SIGN>
+template <typename R, int SIGN, bool PAD = true>
 __device__ __forceinline__ void first_dispatch(const sa_pass_t& a, const cpx<R> (&v)[EMAX],
                                                cpx<R>* line, int tl, bool act) {
   switch (uni(a.radix[0])) {
-    case 2: stage_first_regs<R, 2, SIGN>(v, line, tl, a.T, act); break;
-    case 4: stage_first_regs<R, 4, SIGN>(v, line, tl, a.T, act); break;
-    case 8: stage_first_regs<R, 8, SIGN>(v, line, tl, a.T, act); break;
-    case 16: stage_first_regs<R, 16, SIGN>(v, line, tl, a.T, act); break;
+    case 2: stage_first_regs<R, 2, SIGN, PAD>(v, line, tl, a.T, act); break;
+    case 4: stage_first_regs<R, 4, SIGN, PAD>(v, line, tl, a.T, act); break;
+    case 8: stage_first_regs<R, 8, SIGN, PAD>(v, line, tl, a.T, act); break;
+    case 16: stage_first_regs<R, 16, SIGN, PAD>(v, line, tl, a.T, act); break;
     default: __syncthreads(); break;
   }
 }
 
-template <typename R, int SIGN>
+template <typename R, int SIGN, bool PAD = true>
 __device__ __forceinline__ void last_dispatch(const sa_pass_t& a, cpx<R> (&v)[EMAX],
                                               const cpx<R>* line, int tl, bool act,
                                               const cpx<R>* twb) {
   const int s = a.nstages - 1;
   const cpx<R>* tws = twb + a.twoff[s];
   switch (a.radix[s]) {
-    case 2: stage_last_regs<R, 2, SIGN>(v, line, a.n, tws, tl, a.T, act); break;
-    case 4: stage_last_regs<R, 4, SIGN>(v, line, a.n, tws, tl, a.T, act); break;
-    case 8: stage_last_regs<R, 8, SIGN>(v, line, a.n, tws, tl, a.T, act); break;
-    case 16: stage_last_regs<R, 16, SIGN>(v, line, a.n, tws, tl, a.T, act); break;
+    case 2: stage_last_regs<R, 2, SIGN, PAD>(v, line, a.n, tws, tl, a.T, act); break;
+    case 4: stage_last_regs<R, 4, SIGN, PAD>(v, line, a.n, tws, tl, a.T, act); break;
+    case 8: stage_last_regs<R, 8, SIGN, PAD>(v, line, a.n, tws, tl, a.T, act); break;
+    case 16: stage_last_regs<R, 16, SIGN, PAD>(v, line, a.n, tws, tl, a.T, act); break;
     default:  // not reached: register stages need a radix dividing 16 at both ends
 #pragma unroll
       for (int i = 0; i < EMAX; ++i) v[i] = {R(0), R(0)};
@@ -499,9 +509,10 @@ __device__ __forceinline__ const cpx<R>* filter_of(const sa_pass_t& a, uint32_t 
 // One tile (C consecutive lines) of a pass, run by the whole workgroup.
 // Threads t >= C*T of a larger workgroup idle through the barriers.
 // TWL: the stages read their twiddles from LDS (a.tw_lds > 0).
-template <typename R, int SIGN, int MODE, int MX = 0, bool TWL = false, int E = EMAX>
+template <typename R, int SIGN, int MODE, int MX = 0, bool TWL = false, int E = EMAX, bool PAD = true>
 __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx<R>* lds) {
   static_assert(E == EMAX || MODE != MODE_REGS, "register stages hold EMAX elements");
+  static_assert(PAD || (MODE != MODE_BLUE && MX == 0), "unpadded rows: plain and register-stage passes");
   constexpr bool BLUE = MODE == MODE_BLUE;
 
   const uint32_t nthr = (uint32_t)(a.C * a.T);
@@ -533,7 +544,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
       for (int i = 0; i < E; ++i) v[i] = {R(0), R(0)};
     }
     if constexpr (TWL) twb = stage_twiddles_to_lds<R>(a, lds);
-    first_dispatch<R, SIGN>(a, v, line, (int)tl, active);
+    first_dispatch<R, SIGN, PAD>(a, v, line, (int)tl, active);
   } else {
     const uint32_t lc = slot_line(a, t, a.ld_kfast != 0);
     const uint32_t k0 = slot_elem(a, t, a.ld_kfast != 0);
@@ -550,7 +561,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const uint32_t k = k0 + i * (uint32_t)a.T;
-        if (k < (uint32_t)n) ll[lp((int)k)] = u[i];
+        if (k < (uint32_t)n) ll[lpx<PAD>((int)k)] = u[i];
       }
     }
     __syncthreads();
@@ -560,7 +571,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
   if constexpr (!BLUE) {
     const int s0 = reg_first ? 1 : 0;
     const int s1 = reg_last ? uni(a.nstages) - 1 : uni(a.nstages);
-    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX, E>(a, s, line, stage_lane(tl), lds, twb);
+    for (int s = s0; s < s1; ++s) stage_dispatch<R, SIGN, MX, E, PAD>(a, s, line, stage_lane(tl), lds, twb);
   } else {
     const int nst = uni(a.nstages);
     for (int s = 0; s < nst; ++s) stage_dispatch<R, -1, 0, E>(a, s, line, stage_lane(tl), lds, twb);
@@ -576,7 +587,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
     // are dead after the first stage and must not stay live across the
     // stages (they did when both shared v, and parts of v went to scratch).
     cpx<R> w[E];
-    last_dispatch<R, SIGN>(a, w, line, (int)tl, active, twb);
+    last_dispatch<R, SIGN, PAD>(a, w, line, (int)tl, active, twb);
     const uint32_t b = line0 + c;
     if (active && b < (uint32_t)a.batch) {
       const LineOff o = line_offsets(a, b);
@@ -590,7 +601,7 @@ __device__ __forceinline__ void pass_tile(const sa_pass_t& a, uint32_t tile, cpx
       const cpx<R>* ll = lds + (size_t)sc * a.ldsstride;
       cpx<R> u[E];
 #pragma unroll
-      for (int i = 0; i < E; ++i) u[i] = ll[lp((int)min(k0 + i * (uint32_t)a.T, (uint32_t)n - 1u))];
+      for (int i = 0; i < E; ++i) u[i] = ll[lpx<PAD>((int)min(k0 + i * (uint32_t)a.T, (uint32_t)n - 1u))];
       const LineOff o = line_offsets(a, b);
       store_line<R, SIGN, E>(a, o, k0, u);
     }

@@ -128,7 +128,10 @@ void check_shape(const sa_pass_t& p) {
     if (!g_fixed_launch && (p.n / rad + p.T - 1) / p.T > maxb)
       die("butterflies per thread exceed register budget");
   }
-  if (p.ldsstride < p.n + (p.n - 1) / 16) die("lds stride");
+  if (p.lds_pad != 0 && p.lds_pad != 1) die("lds_pad");
+  if ((p.blu_h || g_fixed_launch || g_blue_emax) && !p.lds_pad) die("unpadded rows outside the generic pass");
+  if (p.ldsstride < (p.lds_pad ? p.n + (p.n - 1) / 16 : p.n)) die("lds stride");
+  if (p.ldsstride % 2 == 0) die("even lds stride");
   if (p.reg_first || p.reg_last) {
     if (p.n != emax * p.T || p.nstages < 2 || p.blu_h) die("register stage on an unsuitable pass");
     auto p2 = [](int r) { return r == 2 || r == 4 || r == 8 || r == 16; };
@@ -293,6 +296,7 @@ extern "C" int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid,
     chk("tw", p->tw, p->tw_lds - 1, prec ? 16 : 8);
   }
   if (p->ldsstride < p->n + (p->n - 1) / 16) die("stride");
+  if (!p->lds_pad) die("fixed kernel rows are padded");
   if (grid < 1) die("grid");
   if (p->pre || p->post || p->blu_h || p->ld_level >= 0 || p->st_level >= 0) die("hooks");
   {
@@ -457,6 +461,8 @@ hipError_t hipMalloc(void** p, size_t n) {
   if (*p) allocs()[(uintptr_t)*p] = n;
   return *p ? hipSuccess : hipErrorOutOfMemory;
 }
+hipError_t hipHostMalloc(void** p, size_t n, unsigned int) { return hipMalloc(p, n); }
+hipError_t hipHostFree(void* p) { return hipFree(p); }
 hipError_t hipFree(void* p) {
   std::lock_guard<std::mutex> lk(allocs_mu());
   allocs().erase((uintptr_t)p);

@@ -87,6 +87,8 @@ def test_served_jobs_match_oracle_and_plain_calls(emu_lib):
     st = srv.stats()
     assert st["completed"] == len(MIX) and st["failed"] == 0
     assert st["plans"] == len({(j[0], _shape(j), j[2]) for j in MIX})  # repeated shape reuses its plan
+    # every job up to 2^20 elements without normalize ran micro-batched
+    assert srv.batch_stats()[1] == sum(1 for j in MIX if not j[4])
     assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
     assert st["arena_peak"] <= st["arena_bytes"]
     want = sum(5 * x.size * np.log2(x.size) for x in xs) * 1e-9
@@ -110,6 +112,42 @@ def test_alone_equals_mixed(emu_lib):
         ya = api.download_ptr(a.ptr, x.size, job[2], L)
         ym = api.download_ptr(m.ptr, x.size, job[2], L)
         assert np.array_equal(ya, ym), job
+
+
+def test_micro_batches_group_jobs(emu_lib):
+    """Jobs pending together are grouped per precision, at most max_jobs a
+    group; with micro-batching off every job runs its own launches; the
+    results are the same either way (the emulation runs both on the host's
+    generic pass)."""
+    L = emu_lib
+    xs, ins = _inputs(L)
+    outs = {}
+    for mode in ("grouped", "off"):
+        srv = api.Server(slots=2, arena_bytes=8 << 20, planners=2, L=L)
+        if mode == "off":
+            srv.set_batching(0)
+        else:
+            srv.set_batching(1 << 20, 3)
+        # plans first, then the whole mix at once so it is pending together
+        warm = [api.DeviceBuffer(x.size, j[2], L) for x, j in zip(xs, MIX)]
+        for job, i, o in zip(MIX, ins, warm):
+            _submit(srv, job, i.ptr, o.ptr)
+        srv.drain()
+        srv.reset_stats()
+        outs[mode] = [api.DeviceBuffer(x.size, j[2], L) for x, j in zip(xs, MIX)]
+        ids = [_submit(srv, job, i.ptr, o.ptr) for job, i, o in zip(MIX, ins, outs[mode])]
+        for jid in ids:
+            assert srv.wait(jid, 120).status == 0
+        b, nj = srv.batch_stats()
+        if mode == "off":
+            assert (b, nj) == (0, 0)
+        else:
+            assert nj == sum(1 for j in MIX if not j[4])
+            assert b >= -(-nj // 3)  # at most 3 jobs a group
+        srv.close()
+    for x, job, a, m in zip(xs, MIX, outs["grouped"], outs["off"]):
+        assert np.array_equal(api.download_ptr(a.ptr, x.size, job[2], L),
+                              api.download_ptr(m.ptr, x.size, job[2], L)), job
 
 
 def test_arena_admission(emu_lib):

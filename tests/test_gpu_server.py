@@ -1,8 +1,10 @@
 """GPU tests of the transform server: every served job gives the same bits
 alone (one slot, one job at a time) and in a mix whose jobs overlap on the
-device, the same bits as the plain backend call, and the oracle's result
-within the precision's tolerance (direct DFT for the smaller jobs; round
-trip for the larger ones)."""
+device, and the oracle's result within the precision's tolerance (direct DFT
+for the smaller jobs; round trip for the larger ones).  Jobs that run their
+own launches give the same bits as the plain backend call; micro-batched
+jobs (up to 2^20 elements, run by the batch kernel) equal it within the
+precision's tolerance, and bit for bit with micro-batching switched off."""
 import numpy as np
 import pytest
 
@@ -24,6 +26,11 @@ MIX = [
     (0, (128, 128, 128), 0, 1, False), (1, 4096, 1, 0, False), (0, (250, 240), 0, 0, True),
 ]
 ORACLE_MAX = 1 << 16
+MB_MAX = 1 << 20  # the server's default micro-batch limit (elements)
+
+
+def _batchable(elems, normalize):
+    return elems <= MB_MAX and not normalize
 
 
 def _shape(job):
@@ -63,6 +70,7 @@ def served(gpu_lib):
     ids = [_submit(mixed, job, i.ptr, o.ptr) for job, i, o in zip(MIX, ins, om)]
     res = [mixed.wait(j, 60) for j in ids]
     stats = mixed.stats()
+    stats["batches"], stats["batched_jobs"] = mixed.batch_stats()
     mixed.close()
     ya = [api.download_ptr(o.ptr, x.size, j[2]).reshape(x.shape) for o, x, j in zip(oa, xs, MIX)]
     ym = [api.download_ptr(o.ptr, x.size, j[2]).reshape(x.shape) for o, x, j in zip(om, xs, MIX)]
@@ -79,6 +87,10 @@ def test_alone_equals_mixed(served):
         assert np.array_equal(a, m), job
     assert all(kept)
     assert stats["completed"] == len(MIX) and stats["failed"] == 0
+    # the small jobs ran micro-batched (how many share a group depends on
+    # how many are pending when the scheduler admits)
+    print(f"micro-batches {stats['batches']}, jobs in them {stats['batched_jobs']}")
+    assert stats["batched_jobs"] >= 6 and stats["batches"] >= 1, stats
 
 
 def test_mix_overlaps_on_device(served):
@@ -104,25 +116,58 @@ def test_served_vs_oracle(served):
         assert oracle.check_rel_error(y, ref, TOL[job[2]]), (job, oracle.rel_l2(y, ref))
 
 
+def _plain_mix(job, x):
+    prec = job[2]
+    if job[0] == 0:
+        p = api.NDBackendPlan(list(job[1]), list(range(len(job[1]))), prec)
+        a, b = api.DeviceBuffer(x.size, prec), api.DeviceBuffer(x.size, prec)
+        a.upload(x.ravel())
+        d, _ = p.execute(a.ptr, b.ptr, job[3])
+        return api.download_ptr(d, x.size, prec).reshape(x.shape)
+    p = api.OneDBackendPlan(job[1], prec)
+    a, b = api.DeviceBuffer(x.size, prec), api.DeviceBuffer(x.size, prec)
+    a.upload(x.ravel())
+    p.execute(a.ptr, b.ptr, job[3])
+    return api.download_ptr(b.ptr, x.size, prec).reshape(x.shape)
+
+
 def test_served_vs_plain_calls(served):
     xs, ya, _, _, _, _ = served
     for x, y, job in zip(xs, ya, MIX):
         if job[4]:
             continue
-        prec = job[2]
-        if job[0] == 0:
-            p = api.NDBackendPlan(list(job[1]), list(range(len(job[1]))), prec)
-            a, b = api.DeviceBuffer(x.size, prec), api.DeviceBuffer(x.size, prec)
-            a.upload(x.ravel())
-            d, _ = p.execute(a.ptr, b.ptr, job[3])
-            z = api.download_ptr(d, x.size, prec).reshape(x.shape)
+        z = _plain_mix(job, x)
+        if _batchable(x.size, job[4]):
+            assert oracle.check_rel_error(y, z, TOL[job[2]]), (job, oracle.rel_l2(y, z))
         else:
-            p = api.OneDBackendPlan(job[1], prec)
-            a, b = api.DeviceBuffer(x.size, prec), api.DeviceBuffer(x.size, prec)
-            a.upload(x.ravel())
-            p.execute(a.ptr, b.ptr, job[3])
-            z = api.download_ptr(b.ptr, x.size, prec).reshape(x.shape)
-        assert np.array_equal(z, y), job
+            assert np.array_equal(z, y), job
+
+
+def test_unbatched_mix_equals_plain_calls(gpu_lib):
+    """Micro-batching off: every job of the mix runs its own launches, in
+    overlap on 8 slots, and gives the plain backend call's bits."""
+    srv = api.Server(slots=8, arena_bytes=1 << 30, planners=2)
+    srv.set_batching(0)
+    xs, ins, outs, ids = [], [], [], []
+    for i, job in enumerate(MIX):
+        x = oracle.uniform_input(_shape(job), 46 + i, DT[job[2]])
+        b = api.DeviceBuffer(x.size, job[2])
+        b.upload(x.ravel())
+        xs.append(x)
+        ins.append(b)
+        outs.append(api.DeviceBuffer(x.size, job[2]))
+    for job, i, o in zip(MIX, ins, outs):
+        ids.append(_submit(srv, job, i.ptr, o.ptr))
+    for jid, job in zip(ids, MIX):
+        r = srv.wait(jid, 60)
+        assert r is not None and r.status == 0, (job, r and r.message)
+    assert srv.batch_stats() == (0, 0)
+    srv.close()
+    for x, job, o in zip(xs, MIX, outs):
+        if job[4]:
+            continue
+        y = api.download_ptr(o.ptr, x.size, job[2]).reshape(x.shape)
+        assert np.array_equal(_plain_mix(job, x), y), job
 
 
 def test_served_round_trip(gpu_lib):
@@ -166,8 +211,9 @@ def test_served_c5_stream_paced(gpu_lib):
     """A draw of the C5 serving stream (Poisson arrivals at 2000 jobs/s,
     sizes 2^10..2^26, 70% complex64) served at its arrival times, with the
     2^25..2^26-element jobs of both precisions in it: every result bit for bit
-    the plain backend call on the same input, and every job's inverse
-    (served too, normalised) back to its input within TOL."""
+    the plain backend call on the same input (within TOL for the
+    micro-batched jobs up to 2^20 elements), and every job's inverse (served
+    too, normalised) back to its input within TOL."""
     import time
 
     reqs = draw_stream(600, seed=46)
@@ -215,6 +261,10 @@ def test_served_c5_stream_paced(gpu_lib):
     srv.close()
     for r, x, o, b in zip(jobs, xs, outs, backs):
         y = api.download_ptr(o.ptr, x.size, r.prec).reshape(x.shape)
-        assert np.array_equal(y, _plain(r, x)), r.describe()
+        z0 = _plain(r, x)
+        if _batchable(x.size, False):
+            assert oracle.check_rel_error(y, z0, TOL[r.prec]), (r.describe(), oracle.rel_l2(y, z0))
+        else:
+            assert np.array_equal(y, z0), r.describe()
         z = api.download_ptr(b.ptr, x.size, r.prec).reshape(x.shape) / np.sqrt(x.size)
         assert oracle.check_rel_error(z, x, TOL[r.prec]), (r.describe(), oracle.rel_l2(z, x))
