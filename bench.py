@@ -533,7 +533,9 @@ def serving_summary(hip, requests=2000, rate=2000.0):
     from shafft_amd.workloads import draw_stream
 
     full = serve_measure(draw_stream(requests, seed=46, rate=rate), hip, full=False)
-    small = serve_measure(draw_stream(requests, seed=46, rate=rate, hi=20), hip, full=False)
+    # up to 256 small jobs outstanding (16 MiB output buffers each): the
+    # saturated client keeps enough requests queued for groups to form
+    small = serve_measure(draw_stream(requests, seed=46, rate=rate, hi=20), hip, outstanding=256, full=False)
     return {
         "workload": f"{requests} jobs, " + STREAM_DESC.format(rate=rate, lo=10, hi=26),
         **{k: full[k] for k in ("p50_ms", "p99_ms", "cold_p50_ms", "cold_p99_ms", "saturated_gflops",
@@ -560,7 +562,8 @@ def bench_serve(args, hip):
     from shafft_amd.workloads import draw_stream
 
     reqs = draw_stream(args.requests, seed=46, rate=args.rate, hi=args.serve_hi)
-    m = serve_measure(reqs, hip, slots=args.slots, arena_gb=args.arena_gb, outstanding=args.outstanding)
+    outstanding = args.outstanding or (256 if args.serve_hi <= 20 else 48)
+    m = serve_measure(reqs, hip, slots=args.slots, arena_gb=args.arena_gb, outstanding=outstanding)
     out = {
         "metric": "served mixed-size stream throughput (GFLOP/s, 5 N log2 N per job); p50/p99 latency",
         "value": m["saturated_gflops"],
@@ -577,7 +580,7 @@ def bench_serve(args, hip):
         "config": {
             "workload": f"{len(reqs)} jobs, " + STREAM_DESC.format(rate=args.rate, lo=10, hi=args.serve_hi),
             "slots": args.slots,
-            "outstanding_buffers": args.outstanding,
+            "outstanding_buffers": outstanding,
             **m,
         },
     }
@@ -696,7 +699,8 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="jobs in the serve workload")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve arrival rate (jobs/s)")
     ap.add_argument("--slots", type=int, default=8, help="server stream slots")
-    ap.add_argument("--outstanding", type=int, default=48, help="serve output buffers (max job size each)")
+    ap.add_argument("--outstanding", type=int, default=0,
+                    help="serve output buffers (max job size each; 0: 256 up to 2^20-element jobs, else 48)")
     ap.add_argument("--arena-gb", type=int, default=24, help="server scratch arena (GiB)")
     ap.add_argument("--serve-hi", type=int, default=26, help="serve: largest job is 2^serve_hi elements")
     ap.add_argument("--no-serving", action="store_true",

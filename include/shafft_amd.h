@@ -201,14 +201,17 @@ int shafftAmdServerDrain(shafft_amd_server* srv);
 int shafftAmdServerStats(shafft_amd_server* srv, int64_t* counts, double* values);
 int shafftAmdServerResetStats(shafft_amd_server* srv);
 /* Micro-batching of small jobs (default: up to 2^20 elements, 64 jobs a
- * group): jobs of at most max_elems elements (0: off) that are admitted in
- * one scheduling round are grouped per precision, up to max_jobs a group,
- * and each group runs as one persistent batch-kernel launch per pass level
- * on one slot.  A grouped job's result is the same alone and in any mix and
- * equals the plain call's within the precision's tolerance (jobs with
- * normalize != 0 and plans with the fixed-length Bluestein kernels always run
- * their own launches).  Applies to jobs admitted after the call. */
-int shafftAmdServerSetBatching(shafft_amd_server* srv, int64_t max_elems, int max_jobs);
+ * group, 2 groups in flight per precision): jobs of at most max_elems
+ * elements (0: off) that are admitted in one scheduling round are grouped
+ * per precision, up to max_jobs a group, and each group runs as one
+ * persistent batch-kernel launch per pass level on one slot; while
+ * max_inflight groups of a precision run, its small jobs wait and gather
+ * into the next group.  A grouped job's result is the same alone and in any
+ * mix and equals the plain call's within the precision's tolerance (jobs
+ * with normalize != 0 and plans with the fixed-length Bluestein kernels
+ * always run their own launches).  Applies to jobs admitted after the call. */
+int shafftAmdServerSetBatching(shafft_amd_server* srv, int64_t max_elems, int max_jobs,
+                               int max_inflight);
 /* Micro-batches formed and the jobs they held since creation or the last
  * shafftAmdServerResetStats. */
 int shafftAmdServerBatchStats(shafft_amd_server* srv, int64_t* batches, int64_t* batched_jobs);

@@ -88,7 +88,7 @@ _SIGS = {
     "shafftAmdServerDrain": (ct.c_int, [ct.c_void_p]),
     "shafftAmdServerStats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_double)]),
     "shafftAmdServerResetStats": (ct.c_int, [ct.c_void_p]),
-    "shafftAmdServerSetBatching": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int]),
+    "shafftAmdServerSetBatching": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int]),
     "shafftAmdServerBatchStats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
     "shafftAmdServerNow": (ct.c_int, [ct.POINTER(ct.c_int64)]),
     "shafftAmdServerDestroy": (ct.c_int, [ct.c_void_p]),
@@ -647,10 +647,11 @@ class Server:
     def reset_stats(self):
         check(self.L.shafftAmdServerResetStats(self.h), "shafftAmdServerResetStats", self.L)
 
-    def set_batching(self, max_elems: int, max_jobs: int = 64):
+    def set_batching(self, max_elems: int, max_jobs: int = 64, max_inflight: int = 2):
         """Micro-batch jobs of at most max_elems elements (0: off), up to
-        max_jobs per group (jobs admitted from now on)."""
-        check(self.L.shafftAmdServerSetBatching(self.h, int(max_elems), int(max_jobs)),
+        max_jobs per group and max_inflight groups running per precision
+        (jobs admitted from now on)."""
+        check(self.L.shafftAmdServerSetBatching(self.h, int(max_elems), int(max_jobs), int(max_inflight)),
               "shafftAmdServerSetBatching", self.L)
 
     def batch_stats(self):

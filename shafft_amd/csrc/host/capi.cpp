@@ -577,10 +577,11 @@ int shafftAmdServerStats(shafft_amd_server* srv, int64_t* counts, double* values
   });
 }
 
-int shafftAmdServerSetBatching(shafft_amd_server* srv, int64_t max_elems, int max_jobs) {
+int shafftAmdServerSetBatching(shafft_amd_server* srv, int64_t max_elems, int max_jobs,
+                               int max_inflight) {
   return guarded([&] {
     if (!srv) fail(ERR_NULLPTR, "null server");
-    srv->be->set_batching(max_elems, max_jobs);
+    srv->be->set_batching(max_elems, max_jobs, max_inflight);
     return 0;
   });
 }

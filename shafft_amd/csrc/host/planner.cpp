@@ -4,6 +4,8 @@
 #include <climits>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <sstream>
 
 #include "fastdiv.hpp"
@@ -330,18 +332,20 @@ static int env_int(const char* name, int dflt) {
 
 static bool pow2_le16(int r) { return r == 2 || r == 4 || r == 8 || r == 16; }
 
-// LDS bank cycles of one tile of the generic pass, summed over every LDS
-// access instruction of every wave, for padded (k -> k + k/16) or unpadded
+// LDS bank cycles of the first wave of a tile of the generic pass (the
+// other waves repeat its lane patterns on other lines and elements), summed
+// over every LDS access instruction, for padded (k -> k + k/16) or unpadded
 // rows of `stride` elements.  Model: `banks` banks of 4 bytes; a wave's
 // access of `w` dwords per lane is served in groups of banks/w lanes, each
-// group taking as many cycles as the most distinct dwords that fall on one
-// bank.  Follows the kernel's lane mappings (pass_tile.hpp): the load's and
-// store's line slots, the butterfly identity of the stages, clamped
-// butterflies in running slots, one butterfly per thread for radices > 16,
-// and the register-resident first and last stages.
+// group taking as many cycles as the most dwords that fall on one bank
+// (lanes repeating the previous lane's element, as clamped butterflies do,
+// count once).  Follows the kernel's lane mappings (pass_tile.hpp): the
+// load's and store's line slots, the butterfly identity of the stages,
+// clamped butterflies in running slots, one butterfly per thread for
+// radices > 16, and the register-resident first and last stages.
 static int64_t lds_bank_cycles(const sa_pass_t& p, int elem_bytes, bool pad, int stride, int banks) {
   const int n = p.n, T = p.T, C = p.C, E = p.emax;
-  const int nthr = C * T;
+  const int nthr = std::min(C * T, 64);
   const int w = elem_bytes / 4;
   const int group = std::max(1, banks / w);
   auto lpk = [pad](int k) { return pad ? k + (k >> 4) : k; };
@@ -358,9 +362,7 @@ static int64_t lds_bank_cycles(const sa_pass_t& p, int elem_bytes, bool pad, int
       int worst = 0;
       for (int l = g0; l < std::min(lanes, g0 + group); ++l) {
         if (!on[l]) continue;
-        bool dup = false;  // same element as an earlier lane of the group: one access
-        for (int m = g0; m < l && !dup; ++m) dup = on[m] && addr[m] == addr[l];
-        if (dup) continue;
+        if (l > g0 && on[l - 1] && addr[l - 1] == addr[l]) continue;  // same element: one access
         for (int d = 0; d < w; ++d) {
           const int b = (int)((addr[l] * w + d) % banks);
           worst = std::max(worst, ++cnt[b]);
@@ -468,13 +470,30 @@ static int64_t lds_bank_cycles(const sa_pass_t& p, int elem_bytes, bool pad, int
 static bool choose_lds_pad(const sa_pass_t& p, int elem_bytes, int padded_stride, int plain_stride) {
   const int knob = env_int("SHAFFT_AMD_LDS_PAD", -1);
   if (knob >= 0) return knob != 0;
-  const double slack = env_int("SHAFFT_AMD_LDS_SLACK_PCT", 110) / 100.0;
+  const int slack_pct = env_int("SHAFFT_AMD_LDS_SLACK_PCT", 110);
+  // Decisions are cached per launch shape (plans of one server repeat them).
+  std::vector<int64_t> key = {p.n, p.T, p.C, p.emax, p.nstages, p.reg_first, p.reg_last, p.ld_kfast,
+                              p.st_kfast, elem_bytes, padded_stride, plain_stride, slack_pct};
+  for (int s = 0; s < p.nstages; ++s) key.push_back(p.radix[s]);
+  static std::mutex mu;
+  static std::map<std::vector<int64_t>, bool> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  bool pad = false;
   for (int banks : {64, 32}) {
     const int64_t cp = lds_bank_cycles(p, elem_bytes, true, padded_stride, banks);
     const int64_t cu = lds_bank_cycles(p, elem_bytes, false, plain_stride, banks);
-    if ((double)cu > slack * (double)cp) return true;
+    if ((double)cu * 100.0 > (double)slack_pct * (double)cp) {
+      pad = true;
+      break;
+    }
   }
-  return false;
+  std::lock_guard<std::mutex> lk(mu);
+  cache.emplace(std::move(key), pad);
+  return pad;
 }
 
 // Fill launch shape: threads per line, lines per workgroup, lane mappings,

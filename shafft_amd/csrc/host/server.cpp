@@ -78,7 +78,8 @@ Server::Server(const Config& cfg) : cfg_(cfg), arena_(cfg.arena_bytes) {
   if (cfg.slots < 1 || cfg.slots > 64) fail(ERR_INVALID_DIM, "slots must be in [1, 64]");
   if (cfg.planners < 1 || cfg.planners > 16) fail(ERR_INVALID_DIM, "planners must be in [1, 16]");
   if (cfg.max_bypass < 0) fail(ERR_INVALID_DIM, "max_bypass must be >= 0");
-  if (cfg.mb_max_elems < 0 || cfg.mb_max_jobs < 1) fail(ERR_INVALID_DIM, "invalid micro-batch limits");
+  if (cfg.mb_max_elems < 0 || cfg.mb_max_jobs < 1 || cfg.mb_max_inflight < 1)
+    fail(ERR_INVALID_DIM, "invalid micro-batch limits");
   hip_check(hipGetDevice(&device_), "hipGetDevice");
   slots_.resize(cfg.slots);
   try {
@@ -257,22 +258,23 @@ void Server::build_mb(Plan& p) const {
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Staging layout of one micro-batch: descriptors, signs, tiles per pass,
-// the pass each pass follows, per-pass counters, queue head + timeout flag,
-// work items.
+// the pass each pass follows, per-pass counters, one queue head per pass
+// level (zero, so no clearing launch runs) + the timeout flag, work items.
 size_t Server::mb_stage_bytes(int64_t np, int64_t items) {
   return align256((size_t)np * sizeof(sa_pass_t)) + align256((size_t)np) + 3 * align256((size_t)np * 4) +
-         256 + (size_t)items * sizeof(sa_batch_item_t);
+         align256((size_t)(np + 1) * 4) + (size_t)items * sizeof(sa_batch_item_t);
 }
 
 bool Server::mb_eligible(const Job& j, const Plan& p) const {
   return p.mb_ok && !j.d.normalize && j.d.elems() <= cfg_.mb_max_elems;
 }
 
-void Server::set_batching(int64_t max_elems, int max_jobs) {
-  if (max_elems < 0 || max_jobs < 1) fail(ERR_INVALID_DIM, "invalid micro-batch limits");
+void Server::set_batching(int64_t max_elems, int max_jobs, int max_inflight) {
+  if (max_elems < 0 || max_jobs < 1 || max_inflight < 1) fail(ERR_INVALID_DIM, "invalid micro-batch limits");
   std::lock_guard<std::mutex> lk(mu_);
   cfg_.mb_max_elems = max_elems;
   cfg_.mb_max_jobs = max_jobs;
+  cfg_.mb_max_inflight = max_inflight;
 }
 
 void Server::release(Job& j) {
@@ -369,8 +371,8 @@ int Server::launch_mb(Slot& s, const std::vector<Job*>& js, std::string& msg) {
     const size_t o_tiles = o_sign + align256((size_t)np);
     const size_t o_dep = o_tiles + align256((size_t)np * 4);
     const size_t o_done = o_dep + align256((size_t)np * 4);
-    const size_t o_head = o_done + align256((size_t)np * 4);
-    const size_t o_items = o_head + 256;
+    const size_t o_head = o_done + align256((size_t)np * 4);  // [level heads..., timeout flag]
+    const size_t o_items = o_head + align256((size_t)(np + 1) * 4);
     sa_pass_t* desc = reinterpret_cast<sa_pass_t*>(h);
     int8_t* sign = reinterpret_cast<int8_t*>(h + o_sign);
     uint32_t* tiles = reinterpret_cast<uint32_t*>(h + o_tiles);
@@ -437,17 +439,19 @@ int Server::launch_mb(Slot& s, const std::vector<Job*>& js, std::string& msg) {
     b.pass_dep = reinterpret_cast<const int32_t*>(dv + o_dep);
     b.wait_ticks = 1;  // independent levels never wait
     b.done = reinterpret_cast<uint32_t*>(dv + o_done);
-    b.head = reinterpret_cast<uint32_t*>(dv + o_head);
-    b.err = b.head + 1;
+    uint32_t* heads = reinterpret_cast<uint32_t*>(dv + o_head);
+    b.err = heads + np;  // levels <= passes
     b.npasses = (uint32_t)np;
     b.independent = 1;
     b.mfma = 0;
+    b.precleared = 1;  // every level has its own zeroed head; `done` is unused when independent
     const sa_batch_item_t* dit = reinterpret_cast<const sa_batch_item_t*>(dv + o_items);
     for (int l = 0; l < maxnp; ++l) {
       const size_t n = level[l + 1] - level[l];
       if (!n) continue;
       b.items = dit + level[l];
       b.nitems = (uint32_t)n;
+      b.head = heads + l;
       const int g = (int)std::min<int64_t>((int64_t)n, cap);
       hip_check((hipError_t)sa_launch_batch(&b, prec, g, block, lds, s.stream), "micro-batch launch");
     }
@@ -502,7 +506,11 @@ void Server::scheduler() {
       dirty_ = false;
       std::vector<Job*> blocked;
       int free_slots = 0;
-      for (const Slot& s : slots_) free_slots += s.job == kSlotFree;
+      int mb_running[2] = {0, 0};  // micro-batches on the device per precision
+      for (const Slot& s : slots_) {
+        free_slots += s.job == kSlotFree;
+        if (s.job == kSlotBatch) ++mb_running[s.mb_prec];
+      }
       int open_mb[2] = {-1, -1};  // micro-batch being formed per precision (slot)
       for (auto it = pending_.begin(); it != pending_.end();) {
         if (free_slots == 0 && open_mb[0] < 0 && open_mb[1] < 0) break;
@@ -538,6 +546,10 @@ void Server::scheduler() {
           ++it;  // needs a slot of its own
           continue;
         }
+        if (mb && open_mb[pr] < 0 && mb_running[pr] >= cfg_.mb_max_inflight) {
+          ++it;  // waits for a running group of its precision: the next group gathers more
+          continue;
+        }
         int64_t off0 = arena_.alloc(p.scratch[0]), off1 = -1;
         if (off0 >= 0) {
           off1 = arena_.alloc(p.scratch[1]);
@@ -567,6 +579,7 @@ void Server::scheduler() {
             o.mb_np = o.mb_items = 0;
             open_mb[pr] = si;
             admitted_mb.push_back(si);
+            ++mb_running[pr];
             ++n_batches_;
           } else {
             slots_[si].job = *it;

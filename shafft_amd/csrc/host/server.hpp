@@ -104,6 +104,7 @@ class Server {
     int max_bypass = 64;
     int64_t mb_max_elems = int64_t(1) << 20;  // micro-batch jobs up to this size (0: off)
     int mb_max_jobs = 64;                     // jobs per micro-batch
+    int mb_max_inflight = 2;                  // micro-batches running per precision
   };
   explicit Server(const Config& cfg);
   ~Server();
@@ -121,8 +122,10 @@ class Server {
   // 0 finished, 1 queued or running; unknown ids are an error.
   int query(int64_t id) const;
   void drain();  // block until nothing is queued or running
-  // Micro-batching limits for jobs admitted from now on (max_elems 0: off).
-  void set_batching(int64_t max_elems, int max_jobs);
+  // Micro-batching limits for jobs admitted from now on (max_elems 0: off;
+  // max_inflight micro-batches of one precision run at once, so that under
+  // load the next group gathers the jobs that arrive meanwhile).
+  void set_batching(int64_t max_elems, int max_jobs, int max_inflight = 2);
   ServeStats stats() const;
   void reset_stats();
   static int64_t now_ns();

@@ -87,10 +87,12 @@ extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int bloc
                                void* stream) {
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const uint32_t n = b->npasses;
-  hipLaunchKernelGGL(sa::batch_clear, dim3((n + 255) / 256 ? (n + 255) / 256 : 1), dim3(256), 0, st,
-                     b->done, n, b->head);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return (int)e;
+  if (!b->precleared) {
+    hipLaunchKernelGGL(sa::batch_clear, dim3((n + 255) / 256 ? (n + 255) / 256 : 1), dim3(256), 0, st,
+                       b->done, n, b->head);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+  }
   // 160 KiB minus room for the kernel's static LDS (the queue item slot);
   // set once per variant (thread-safe static init).
   static const bool attr = [] {

@@ -150,11 +150,15 @@ typedef struct {
   /* Radix-8/16 stages of complex-float passes: 0 FP32 on the vector ALUs,
    * 1 bfloat16 hi/lo split on the matrix cores, 2 float16 hi/lo split. */
   int32_t mfma;
+  /* 1: `head` and `done` are already zero for this launch (the caller
+   * wrote them), so no clearing kernel runs before it.                     */
+  int32_t precleared;
 } sa_batch_t;
 
 /* Zero the queue head and the per-pass counters (a kernel, so it can sit in
- * a captured graph), then run the persistent batch kernel with `grid`
- * workgroups of `block` threads and `lds` bytes of dynamic LDS. */
+ * a captured graph; skipped when b->precleared), then run the persistent
+ * batch kernel with `grid` workgroups of `block` threads and `lds` bytes of
+ * dynamic LDS. */
 int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int block, size_t lds, void* stream);
 /* Resident batch workgroups per CU for a block size and LDS size. */
 int sa_batch_occupancy(int prec, int block, size_t lds);
