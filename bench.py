@@ -459,7 +459,7 @@ def _plain_run(reqs, hip, src, dst, with_planning=False):
     return time.perf_counter() - t0
 
 
-def serve_measure(reqs, hip, slots=8, arena_gb=24, outstanding=48, full=True):
+def serve_measure(reqs, hip, slots=8, arena_gb=24, outstanding=48, full=True, planners=8):
     """Serves `reqs` (each job reads a shared input and writes one of
     `outstanding` output buffers): a cold paced run that builds the plans,
     a paced run from resident plans (latency), every job queued at once
@@ -470,7 +470,7 @@ def serve_measure(reqs, hip, slots=8, arena_gb=24, outstanding=48, full=True):
     max_e, inp = _serve_inputs(reqs)
     outs = [api.DeviceBuffer(max_e, api.Z2Z) for _ in range(outstanding)]
     gflop = sum(r.gflop for r in reqs)
-    srv = api.Server(slots=slots, arena_bytes=arena_gb << 30, planners=2)
+    srv = api.Server(slots=slots, arena_bytes=arena_gb << 30, planners=planners)
     cold = _lat_stats(_serve_run(srv, reqs, inp, outs, paced=True))
     st_cold = srv.stats()
     srv.reset_stats()
@@ -512,7 +512,7 @@ def serve_measure(reqs, hip, slots=8, arena_gb=24, outstanding=48, full=True):
         "arena_peak_gb": round(st["arena_peak"] / 2**30, 3),
     }
     if full:
-        srv = api.Server(slots=slots, arena_bytes=arena_gb << 30, planners=2)
+        srv = api.Server(slots=slots, arena_bytes=arena_gb << 30, planners=planners)
         cold_sat = _lat_stats(_serve_run(srv, reqs, inp, outs, paced=False))
         srv.close()
         plain_planned = _plain_run(reqs, hip, inp[api.Z2Z], outs[0], with_planning=True)
@@ -532,7 +532,7 @@ def serving_summary(hip, requests=2000, rate=2000.0):
     per-call overheads rather than HBM bandwidth bound the plain path."""
     from shafft_amd.workloads import draw_stream
 
-    full = serve_measure(draw_stream(requests, seed=46, rate=rate), hip, full=False)
+    full = serve_measure(draw_stream(requests, seed=46, rate=rate), hip, full=True)
     # up to 256 small jobs outstanding (16 MiB output buffers each): the
     # saturated client keeps enough requests queued for groups to form
     small = serve_measure(draw_stream(requests, seed=46, rate=rate, hi=20), hip, outstanding=256, full=False)
@@ -540,7 +540,9 @@ def serving_summary(hip, requests=2000, rate=2000.0):
         "workload": f"{requests} jobs, " + STREAM_DESC.format(rate=rate, lo=10, hi=26),
         **{k: full[k] for k in ("p50_ms", "p99_ms", "cold_p50_ms", "cold_p99_ms", "saturated_gflops",
                                 "saturated_req_per_s", "plain_per_call_gflops", "speedup_vs_plain",
-                                "gpu_busy_frac", "micro_batched_jobs", "micro_batches")},
+                                "gpu_busy_frac", "micro_batched_jobs", "micro_batches", "cold_plan_builds",
+                                "cold_saturated_s", "plain_with_planning_s",
+                                "cold_speedup_vs_plain_with_planning")},
         "target_speedup_vs_plain": 3.0,
         "substream_le_2p20": {
             "workload": f"{requests} jobs, " + STREAM_DESC.format(rate=rate, lo=10, hi=20),
@@ -563,7 +565,8 @@ def bench_serve(args, hip):
 
     reqs = draw_stream(args.requests, seed=46, rate=args.rate, hi=args.serve_hi)
     outstanding = args.outstanding or (256 if args.serve_hi <= 20 else 48)
-    m = serve_measure(reqs, hip, slots=args.slots, arena_gb=args.arena_gb, outstanding=outstanding)
+    m = serve_measure(reqs, hip, slots=args.slots, arena_gb=args.arena_gb, outstanding=outstanding,
+                      planners=args.planners)
     out = {
         "metric": "served mixed-size stream throughput (GFLOP/s, 5 N log2 N per job); p50/p99 latency",
         "value": m["saturated_gflops"],
@@ -580,6 +583,7 @@ def bench_serve(args, hip):
         "config": {
             "workload": f"{len(reqs)} jobs, " + STREAM_DESC.format(rate=args.rate, lo=10, hi=args.serve_hi),
             "slots": args.slots,
+            "planner_threads": args.planners,
             "outstanding_buffers": outstanding,
             **m,
         },
@@ -699,6 +703,7 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="jobs in the serve workload")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve arrival rate (jobs/s)")
     ap.add_argument("--slots", type=int, default=8, help="server stream slots")
+    ap.add_argument("--planners", type=int, default=8, help="server plan-building threads")
     ap.add_argument("--outstanding", type=int, default=0,
                     help="serve output buffers (max job size each; 0: 256 up to 2^20-element jobs, else 48)")
     ap.add_argument("--arena-gb", type=int, default=24, help="server scratch arena (GiB)")

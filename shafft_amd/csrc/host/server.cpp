@@ -81,6 +81,8 @@ Server::Server(const Config& cfg) : cfg_(cfg), arena_(cfg.arena_bytes) {
   if (cfg.mb_max_elems < 0 || cfg.mb_max_jobs < 1 || cfg.mb_max_inflight < 1)
     fail(ERR_INVALID_DIM, "invalid micro-batch limits");
   hip_check(hipGetDevice(&device_), "hipGetDevice");
+  if (hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_) != hipSuccess || cus_ < 1)
+    cus_ = 256;
   slots_.resize(cfg.slots);
   try {
     for (Slot& s : slots_) {
@@ -427,11 +429,7 @@ int Server::launch_mb(Slot& s, const std::vector<Job*>& js, std::string& msg) {
     auto oc = occ_cache_.find(key);
     if (oc == occ_cache_.end())
       oc = occ_cache_.emplace(key, std::max(1, sa_batch_occupancy(prec, block, lds))).first;
-    int cus = 0, dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-      cus = 256;
-    const int64_t cap = (int64_t)oc->second * cus;
+    const int64_t cap = (int64_t)oc->second * cus_;
     sa_batch_t b{};
     b.passes = reinterpret_cast<const sa_pass_t*>(dv);
     b.pass_sign = reinterpret_cast<const int8_t*>(dv + o_sign);

@@ -203,6 +203,7 @@ class Server {
   int64_t next_id_ = 1;
   int running_ = 0;
   int device_ = 0;
+  int cus_ = 256;  // compute units of the device (persistent grid sizes)
   bool stop_ = false;
   bool dirty_ = false;  // queue, plans or slots changed since the last admission scan
   std::map<std::pair<int, std::pair<int, size_t>>, int> occ_cache_;  // scheduler thread
