@@ -38,9 +38,10 @@ __device__ __forceinline__ uint32_t tile_of_block(const sa_pass_t& a) {
 // (complex float, radices up to 21) / ~151 (complex double, up to 28) VGPRs,
 // 5 / 3 waves per SIMD.  The planner's tiles hold at most 64 KiB of data
 // (512 threads complex float, 256 complex double), so several independent
-// tiles share a CU and one's HBM loads overlap another's LDS stages; the
-// stages themselves are VALU-issue bound (~120 VALU instructions per element
-// over 3 stages, load and store, by SQ counters).
+// tiles share a CU and one's HBM loads overlap another's LDS stages (the
+// stages issue ~100-160 VALU instructions per element over 3 stages, load
+// and store, by SQ counters; how far the two workgroups' memory and LDS
+// phases overlap sets the pass time more than the instruction count).
 // E: elements per thread (8 only for the in-LDS Bluestein rows of the
 // three-pass long Bluestein: twice the threads on a pass of few, long rows).
 // PAD: LDS rows padded (k -> k + k/16); unpadded for passes whose accesses
