@@ -1,6 +1,7 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <sstream>
 
@@ -28,6 +29,14 @@ static uint64_t wall_clock_hz() {
     return (uint64_t)100000000ull;  // gfx9 wall clock: 100 MHz
   }();
   return cached;
+}
+
+uint32_t batch_level_queues(size_t nitems) {
+  static const bool one = [] {
+    const char* e = std::getenv("SHAFFT_AMD_BATCH_QUEUES");
+    return e && std::atoi(e) == 1;
+  }();
+  return (!one && nitems >= 2048) ? 8u : 1u;
 }
 
 bool batch_retile(sa_pass_t& p, int elem_bytes, int block, int lds_budget) {
@@ -191,7 +200,7 @@ void BatchPlan::build() {
   up(d_items_, items_.data(), items_.size() * sizeof(sa_batch_item_t));
   if (d_done_.bytes < std::max<size_t>(np, 1) * 4) d_done_ = DevBuf(std::max<size_t>(np, 1) * 4);
   if (!d_head_.p) {
-    d_head_ = DevBuf(256);  // [0] queue head, [1] wait-timeout flag (sticky)
+    d_head_ = DevBuf(256);  // [0..7] queue heads, [8] wait-timeout flag (sticky)
     hip_check(hipMemset(d_head_.p, 0, 256), "hipMemset(batch flag)");
   }
   built_ = true;
@@ -219,7 +228,7 @@ void BatchPlan::launch(int dir, hipStream_t s) {
   b.pass_sign = static_cast<const int8_t*>(d_sign_[d].p);
   b.done = static_cast<uint32_t*>(d_done_.p);
   b.head = static_cast<uint32_t*>(d_head_.p);
-  b.err = static_cast<uint32_t*>(d_head_.p) + 1;
+  b.err = static_cast<uint32_t*>(d_head_.p) + 8;
   // Two seconds of the device wall clock: far beyond any legitimate wait
   // for one pass of one transform.
   b.wait_ticks = 2ull * wall_clock_hz();
@@ -230,16 +239,19 @@ void BatchPlan::launch(int dir, hipStream_t s) {
     b.items = all;
     b.nitems = (uint32_t)items_.size();
     b.independent = 0;
+    b.queues = 1;  // dependency waits need the list's order: one queue
     hip_check((hipError_t)sa_launch_batch(&b, prec_, grid_, block_, lds_, s), "batch launch");
     return;
   }
-  // Level mode: one persistent launch per pass level, in stream order.
+  // Level mode: one persistent launch per pass level, in stream order, one
+  // work queue per XCD.
   b.independent = 1;
   for (size_t l = 0; l + 1 < level_start_.size(); ++l) {
     const size_t n = level_start_[l + 1] - level_start_[l];
     if (!n) continue;
     b.items = all + level_start_[l];
     b.nitems = (uint32_t)n;
+    b.queues = batch_level_queues(n);
     const int g = (int)std::min<int64_t>((int64_t)n, grid_);
     hip_check((hipError_t)sa_launch_batch(&b, prec_, g, block_, lds_, s), "batch launch");
   }
@@ -286,7 +298,7 @@ void BatchPlan::execute(int dir, bool use_graph) {
 // launch fails exactly one synchronize.
 void BatchPlan::check_timeout_flag() {
   if (!d_head_.p) return;
-  uint32_t* flag = static_cast<uint32_t*>(d_head_.p) + 1;
+  uint32_t* flag = static_cast<uint32_t*>(d_head_.p) + 8;
   uint32_t err = 0;
   hip_check(hipMemcpy(&err, flag, 4, hipMemcpyDeviceToHost), "hipMemcpy(batch flag)");
   if (err) {

@@ -35,6 +35,14 @@ bool batch_retile(sa_pass_t& p, int elem_bytes, int block, int lds_budget);
 // slot sits in static LDS).
 constexpr int kBatchLdsBudget = 159 * 1024;
 
+// Work queues of an independent (level-mode) batch launch of `nitems`
+// items: 8, one per XCD, for launches of at least 2048 items; 1 for smaller
+// ones (a workgroup that runs out of items tries every queue before it
+// exits, which costs more than the L2 locality returns on a short launch:
+// micro-batches of the serving stream's small jobs 137k -> 82k jobs/s) and
+// always with SHAFFT_AMD_BATCH_QUEUES=1 (read once).
+uint32_t batch_level_queues(size_t nitems);
+
 class BatchPlan {
  public:
   BatchPlan(int count, const int* ndims, const int* dims_flat, int prec);

@@ -260,11 +260,11 @@ void Server::build_mb(Plan& p) const {
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Staging layout of one micro-batch: descriptors, signs, tiles per pass,
-// the pass each pass follows, per-pass counters, one queue head per pass
+// the pass each pass follows, per-pass counters, eight queue heads per pass
 // level (zero, so no clearing launch runs) + the timeout flag, work items.
 size_t Server::mb_stage_bytes(int64_t np, int64_t items) {
   return align256((size_t)np * sizeof(sa_pass_t)) + align256((size_t)np) + 3 * align256((size_t)np * 4) +
-         align256((size_t)(np + 1) * 4) + (size_t)items * sizeof(sa_batch_item_t);
+         align256((size_t)(8 * np + 1) * 4) + (size_t)items * sizeof(sa_batch_item_t);
 }
 
 bool Server::mb_eligible(const Job& j, const Plan& p) const {
@@ -373,8 +373,8 @@ int Server::launch_mb(Slot& s, const std::vector<Job*>& js, std::string& msg) {
     const size_t o_tiles = o_sign + align256((size_t)np);
     const size_t o_dep = o_tiles + align256((size_t)np * 4);
     const size_t o_done = o_dep + align256((size_t)np * 4);
-    const size_t o_head = o_done + align256((size_t)np * 4);  // [level heads..., timeout flag]
-    const size_t o_items = o_head + align256((size_t)(np + 1) * 4);
+    const size_t o_head = o_done + align256((size_t)np * 4);  // [8 heads per level..., timeout flag]
+    const size_t o_items = o_head + align256((size_t)(8 * np + 1) * 4);
     sa_pass_t* desc = reinterpret_cast<sa_pass_t*>(h);
     int8_t* sign = reinterpret_cast<int8_t*>(h + o_sign);
     uint32_t* tiles = reinterpret_cast<uint32_t*>(h + o_tiles);
@@ -438,18 +438,20 @@ int Server::launch_mb(Slot& s, const std::vector<Job*>& js, std::string& msg) {
     b.wait_ticks = 1;  // independent levels never wait
     b.done = reinterpret_cast<uint32_t*>(dv + o_done);
     uint32_t* heads = reinterpret_cast<uint32_t*>(dv + o_head);
-    b.err = heads + np;  // levels <= passes
+    b.err = heads + 8 * np;  // levels <= passes
     b.npasses = (uint32_t)np;
     b.independent = 1;
     b.mfma = 0;
-    b.precleared = 1;  // every level has its own zeroed head; `done` is unused when independent
+    b.precleared = 1;  // every level has its own zeroed heads; `done` is unused when independent
+
     const sa_batch_item_t* dit = reinterpret_cast<const sa_batch_item_t*>(dv + o_items);
     for (int l = 0; l < maxnp; ++l) {
       const size_t n = level[l + 1] - level[l];
       if (!n) continue;
       b.items = dit + level[l];
       b.nitems = (uint32_t)n;
-      b.head = heads + l;
+      b.queues = batch_level_queues(n);  // per-XCD queues on long levels
+      b.head = heads + 8 * l;
       const int g = (int)std::min<int64_t>((int64_t)n, cap);
       hip_check((hipError_t)sa_launch_batch(&b, prec, g, block, lds, s.stream), "micro-batch launch");
     }

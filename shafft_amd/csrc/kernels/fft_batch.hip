@@ -23,10 +23,10 @@ namespace sa {
 // Clears the per-pass completion counters and the work-queue head.  The
 // wait-timeout flag is not cleared here: it stays set across launches (and
 // graph replays) until the host has read it.
-__global__ void batch_clear(uint32_t* done, uint32_t n, uint32_t* head) {
+__global__ void batch_clear(uint32_t* done, uint32_t n, uint32_t* head, uint32_t queues) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) done[i] = 0u;
-  if (i == 0) *head = 0u;
+  if (i < queues) head[i] = 0u;
 }
 
 template <typename R, int MX>
@@ -36,13 +36,25 @@ __global__ __launch_bounds__(512) void fft_batch_kernel(const sa_batch_t b) {
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(lds_raw);
   // Every branch and loop bound below that encloses a barrier is made
   // wave-uniform with readfirstlane (see uni() in pass_tile.hpp).
+  // Queue q holds items [q * per, min((q + 1) * per, nitems)); a workgroup
+  // starts on its XCD's queue and moves on to the next when that is used up,
+  // so the loop ends after at most nitems + queues taken counters.
+  const uint32_t nq = b.queues > 1 ? b.queues : 1u;
+  const uint32_t per = (b.nitems + nq - 1) / nq;
+  uint32_t q = nq > 1 ? (blockIdx.x % nq) : 0u, tried = 0;
   for (;;) {
     if (threadIdx.x == 0)
-      s_item = __hip_atomic_fetch_add(b.head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_item = __hip_atomic_fetch_add(b.head + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const uint32_t it = (uint32_t)uni((int)s_item);
+    const uint32_t k = (uint32_t)uni((int)s_item);
     __syncthreads();
-    if (it >= b.nitems) break;
+    const uint32_t lo = q * per, hi = min(lo + per, b.nitems);
+    if (lo + k >= hi) {  // this queue is used up
+      if (++tried >= nq) break;
+      q = q + 1 == nq ? 0u : q + 1;
+      continue;
+    }
+    const uint32_t it = lo + k;
     const uint32_t pass = (uint32_t)uni((int)b.items[it].pass);
     const uint32_t tile = (uint32_t)uni((int)b.items[it].tile);
     const int dep = b.independent ? -1 : uni(b.pass_dep[pass]);
@@ -87,9 +99,11 @@ extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int bloc
                                void* stream) {
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const uint32_t n = b->npasses;
+  const uint32_t nq = b->queues > 1 ? b->queues : 1u;
+  if (nq != 1 && (nq != 8 || !b->independent)) return (int)hipErrorInvalidValue;  // see sa_batch_t
   if (!b->precleared) {
-    hipLaunchKernelGGL(sa::batch_clear, dim3((n + 255) / 256 ? (n + 255) / 256 : 1), dim3(256), 0, st,
-                       b->done, n, b->head);
+    const uint32_t m = n > nq ? n : nq;
+    hipLaunchKernelGGL(sa::batch_clear, dim3((m + 255) / 256), dim3(256), 0, st, b->done, n, b->head, nq);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
   }

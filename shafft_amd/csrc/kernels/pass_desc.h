@@ -153,6 +153,12 @@ typedef struct {
   /* 1: `head` and `done` are already zero for this launch (the caller
    * wrote them), so no clearing kernel runs before it.                     */
   int32_t precleared;
+  /* Work queues (1 or 8; 8 only with `independent`): the list is split into
+   * `queues` contiguous ranges, range q taken by the workgroups of XCD q
+   * (workgroup b runs on XCD b % 8) through its own counter head[q], so
+   * neighbouring tiles meet in one L2; a workgroup whose range is used up
+   * takes items from the other ranges.  `head` holds `queues` counters.    */
+  uint32_t queues;
 } sa_batch_t;
 
 /* Zero the queue head and the per-pass counters (a kernel, so it can sit in

@@ -414,6 +414,7 @@ extern "C" int sa_launch_batch(const sa_batch_t* bp, int prec, int grid, int blo
     };
     if (grid < 1 || block < 1 || block > 512 || lds > 159 * 1024) die("launch shape");
     if (!b.err || b.wait_ticks == 0) die("missing wait bound");
+    if (b.queues != 1 && (b.queues != 8 || !b.independent)) die("work queues");
     bool first_level = true;
     for (uint32_t it = 0; it < b.nitems; ++it)
       if (b.pass_dep[b.items[it].pass] >= 0) first_level = false;
