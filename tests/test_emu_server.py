@@ -150,6 +150,43 @@ def test_micro_batches_group_jobs(emu_lib):
                               api.download_ptr(m.ptr, x.size, job[2], L)), job
 
 
+@pytest.mark.parametrize("max_jobs,max_inflight", [(1, 1), (2, 1), (64, 2), (5, 8)])
+def test_micro_batch_limits(emu_lib, max_jobs, max_inflight):
+    """Every combination of group size and groups in flight serves the whole
+    mix (normalised jobs, which never group, included) with the results of
+    the server with micro-batching off; no group exceeds max_jobs."""
+    L = emu_lib
+    xs, ins = _inputs(L)
+    res = {}
+    for mode in ("off", "on"):
+        srv = api.Server(slots=3, arena_bytes=8 << 20, planners=2, L=L)
+        if mode == "off":
+            srv.set_batching(0)
+        else:
+            srv.set_batching(1 << 20, max_jobs, max_inflight)
+        outs = [api.DeviceBuffer(x.size, j[2], L) for x, j in zip(xs, MIX)]
+        ids = [_submit(srv, job, i.ptr, o.ptr) for job, i, o in zip(MIX, ins, outs)]
+        for jid in ids:
+            r = srv.wait(jid, 120)
+            assert r is not None and r.status == 0, r and r.message
+        b, nj = srv.batch_stats()
+        if mode == "on":
+            assert nj == sum(1 for j in MIX if not j[4])
+            assert b * max_jobs >= nj
+        srv.close()
+        res[mode] = [api.download_ptr(o.ptr, x.size, j[2], L) for o, x, j in zip(outs, xs, MIX)]
+    for a, m, job in zip(res["off"], res["on"], MIX):
+        assert np.array_equal(a, m), job
+
+
+def test_micro_batch_limits_rejected(emu_lib):
+    srv = api.Server(slots=1, arena_bytes=1 << 20, planners=1, L=emu_lib)
+    for args in ((-1, 64, 2), (1 << 20, 0, 2), (1 << 20, 64, 0)):
+        with pytest.raises(api.ShafftError):
+            srv.set_batching(*args)
+    srv.close()
+
+
 def test_arena_admission(emu_lib):
     L = emu_lib
     # (64,64) complex128 needs one 64 KiB scratch block; (4096,) 1-D in one pass needs none.
