@@ -176,7 +176,9 @@ int shafftAmdBatchDestroy(shafft_amd_batch_plan* plan);
  * buffers of the job's element count, distinct, valid until the job ends.
  * direction: 0 forward, 1 backward; normalize != 0 scales the output by
  * 1/sqrt(elements).  arrival_ns < 0 stamps the submit time; otherwise it is
- * a shafftAmdServerNow() time from which latency is counted. */
+ * a shafftAmdServerNow() time from which latency is counted.  One slot per
+ * hardware queue the HIP runtime gives the process (GPU_MAX_HW_QUEUES, 4 by
+ * default) serves best: streams beyond that share queues and serialise. */
 typedef struct shafft_amd_server shafft_amd_server;
 int shafftAmdServerCreate(shafft_amd_server** srv, int slots, int64_t arena_bytes,
                           int planner_threads, int max_bypass);

@@ -582,7 +582,7 @@ class Server:
     include/shafft_amd.h): jobs of mixed shape and precision are admitted
     onto `slots` device streams as slots and arena scratch free up."""
 
-    def __init__(self, slots: int = 8, arena_bytes: int = 16 << 30, planners: int = 2, max_bypass: int = 64,
+    def __init__(self, slots: int = 4, arena_bytes: int = 16 << 30, planners: int = 2, max_bypass: int = 64,
                  L=None):
         self.L = L or _L()
         self.h = ct.c_void_p()

@@ -98,7 +98,11 @@ class Arena {
 class Server {
  public:
   struct Config {
-    int slots = 8;
+    // Stream slots: one per hardware queue the HIP runtime gives the process
+    // (GPU_MAX_HW_QUEUES, 4 by default); streams beyond that share queues in
+    // pairs and the jobs on them serialise (serving stream: 4 slots 1.39-1.42x
+    // the plain path, 8 slots 1.29-1.33x, 2 slots 1.31x).
+    int slots = 4;
     int64_t arena_bytes = int64_t(16) << 30;
     int planners = 2;
     int max_bypass = 64;
