@@ -219,6 +219,10 @@ def bench_batch(args, hip):
         if name not in wanted:
             continue
         bp.set_precision_mode(mode)
+        # each mode runs the schedule the autotuner picks for it (the split
+        # stages use more registers than the FP32 ones)
+        mode_tune = bp.autotune() if not args.no_autotune else ""
+        mode_sched = mode_tune.strip().splitlines()[-1] if mode_tune else "defaults"
         for _ in range(args.warmup):
             bp.execute(api.FORWARD)
         bp.synchronize()
@@ -233,9 +237,12 @@ def bench_batch(args, hip):
             y = api.download_ptr(o.ptr + bp.offsets[i] * 8, ref.size, prec).reshape(ref.shape)
             errs.append(float(np.linalg.norm(y - ref) / np.linalg.norm(ref)))
         modes[name] = {"gpu_ms": round(mms, 4), "gflops": round(flops / (mms * 1e-3) / 1e9, 1),
+                       "schedule": mode_sched,
                        "relL2_vs_fp64_median": float(f"{np.median(errs):.3e}"),
                        "relL2_vs_fp64_max": float(f"{np.max(errs):.3e}")}
     bp.set_precision_mode(api.FP32)
+    if not args.no_autotune and len(wanted) > 1:
+        bp.autotune()  # back to the FP32 schedule for the plain-path comparison
     if args.skip_plain:
         plain = float("nan")
     else:

@@ -122,6 +122,11 @@ extern "C" int sa_launch_batch(const sa_batch_t* b, int prec, int grid, int bloc
   (void)attr;
   if (prec && b->mfma) return (int)hipErrorInvalidValue;  // split stages are complex-float only
   if ((block & 63) != 0) return (int)hipErrorInvalidValue;  // matrix-core stages need whole waves
+  if (b->mfma) {  // the stages' constant operands, ahead of the kernel on its stream
+    hipLaunchKernelGGL(sa::mx_table_init, dim3(1), dim3(64), 0, st);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+  }
   const dim3 g((uint32_t)grid), t((uint32_t)block);
   if (prec)
     hipLaunchKernelGGL((sa::fft_batch_kernel<double, 0>), g, t, lds, st, *b);

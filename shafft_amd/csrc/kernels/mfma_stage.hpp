@@ -89,6 +89,72 @@ __device__ __forceinline__ double mx_cos16(int j) {
   return neg ? -v : v;
 }
 
+// The constant DFT operands of every (mode, radix, direction), built once
+// by mx_table_init and loaded by the stages (each stage call built its own
+// before: ~16 double-precision cosines, selects and conversions per lane per
+// operand vector).  Slot (mode MX, radix index ri (0: 8, 1: 16), direction
+// index si (0: forward), 16-row block b, half hl (0: hi, 1: lo), lane) holds
+// the lane's KPL 16-bit values in its first 2 * KPL bytes.  Per translation
+// unit (no relocatable device code); the batch launch runs mx_table_init on
+// its stream before a split-precision batch kernel.
+constexpr int kMxSlots = 2 * 2 * 2 * 2 * 2 * 64;
+static __device__ __align__(16) uint4 g_mx_tab[kMxSlots];
+
+__device__ __forceinline__ int mx_slot(int mx, int ri, int si, int b, int hl, int lane) {
+  return (((((mx - 1) * 2 + ri) * 2 + si) * 2 + b) * 2 + hl) * 64 + lane;
+}
+
+// Operand vectors of one (MX, RAD, SIGN) for this lane.
+template <int MX, int RAD, int SIGN>
+__device__ __forceinline__ void mx_build(int lane, typename MxVec<MX, RAD == 16 ? 8 : 4>::type (&ahi)[RAD / 8],
+                                         typename MxVec<MX, RAD == 16 ? 8 : 4>::type (&alo)[RAD / 8]) {
+  constexpr int KPL = RAD == 16 ? 8 : 4;
+  constexpr int MB = RAD / 8;
+  constexpr float LO = MX == 2 ? 2048.f : 1.f;
+  typedef MxVec<MX, KPL> V;
+  const int j = lane & 15, h = lane >> 4;
+#pragma unroll
+  for (int b = 0; b < MB; ++b) {
+    const int row = 16 * b + j, p = row >> 1, cp = row & 1;
+#pragma unroll
+    for (int e = 0; e < KPL; ++e) {
+      const int k = KPL * h + e, m = k >> 1, cc = k & 1;
+      const int q = (p * m) % RAD;
+      const double wr = mx_cos16(q * (16 / RAD));
+      const double wi = SIGN * mx_cos16(q * (16 / RAD) - 4);  // sin
+      const double val = cp == 0 ? (cc == 0 ? wr : -wi) : (cc == 0 ? wi : wr);
+      V::set(ahi[b], e, (float)val);
+      V::set(alo[b], e, (float)((val - (double)V::get(ahi[b], e)) * LO));
+    }
+  }
+}
+
+template <int MX, int RAD, int SIGN>
+__device__ __forceinline__ void mx_store(int lane) {
+  typedef typename MxVec<MX, RAD == 16 ? 8 : 4>::type vec;
+  vec ahi[RAD / 8], alo[RAD / 8];
+  mx_build<MX, RAD, SIGN>(lane, ahi, alo);
+  const int ri = RAD == 16 ? 1 : 0, si = SIGN < 0 ? 0 : 1;
+#pragma unroll
+  for (int b = 0; b < RAD / 8; ++b) {
+    *reinterpret_cast<vec*>(&g_mx_tab[mx_slot(MX, ri, si, b, 0, lane)]) = ahi[b];
+    *reinterpret_cast<vec*>(&g_mx_tab[mx_slot(MX, ri, si, b, 1, lane)]) = alo[b];
+  }
+}
+
+// One workgroup of 64 threads fills every slot.
+static __global__ __launch_bounds__(64) void mx_table_init() {
+  const int lane = (int)threadIdx.x;
+  mx_store<1, 16, -1>(lane);
+  mx_store<1, 16, 1>(lane);
+  mx_store<1, 8, -1>(lane);
+  mx_store<1, 8, 1>(lane);
+  mx_store<2, 16, -1>(lane);
+  mx_store<2, 16, 1>(lane);
+  mx_store<2, 8, -1>(lane);
+  mx_store<2, 8, 1>(lane);
+}
+
 // One stage (radix RAD = 8 or 16, Ns = product of the earlier radices) over
 // all C line slots of the workgroup's LDS tile; every thread of the
 // workgroup (whole waves) takes part.  Same data movement as stage_lds:
@@ -115,20 +181,15 @@ __device__ __forceinline__ void stage_mfma(cpx<float>* lds, int C, int stride, i
   const int nbt = C * nb;
   const FastDiv fd{nsd.d, nsd.m, nsd.s};
 
-  // Constant DFT operand (rows 16b + (lane & 15), columns KPL*h + e).
+  // Constant DFT operand (rows 16b + (lane & 15), columns KPL*h + e), from
+  // the table mx_table_init filled.
   vec ahi[MB], alo[MB];
+  {
+    const int ri = RAD == 16 ? 1 : 0, si = SIGN < 0 ? 0 : 1;
 #pragma unroll
-  for (int b = 0; b < MB; ++b) {
-    const int row = 16 * b + j, p = row >> 1, cp = row & 1;
-#pragma unroll
-    for (int e = 0; e < KPL; ++e) {
-      const int k = KPL * h + e, m = k >> 1, cc = k & 1;
-      const int q = (p * m) % RAD;
-      const double wr = mx_cos16(q * (16 / RAD));
-      const double wi = SIGN * mx_cos16(q * (16 / RAD) - 4);  // sin
-      const double val = cp == 0 ? (cc == 0 ? wr : -wi) : (cc == 0 ? wi : wr);
-      V::set(ahi[b], e, (float)val);
-      V::set(alo[b], e, (float)((val - (double)V::get(ahi[b], e)) * LO));
+    for (int b = 0; b < MB; ++b) {
+      ahi[b] = *reinterpret_cast<const vec*>(&g_mx_tab[mx_slot(MX, ri, si, b, 0, lane)]);
+      alo[b] = *reinterpret_cast<const vec*>(&g_mx_tab[mx_slot(MX, ri, si, b, 1, lane)]);
     }
   }
 
