@@ -268,3 +268,49 @@ def test_served_c5_stream_paced(gpu_lib):
             assert np.array_equal(y, z0), r.describe()
         z = api.download_ptr(b.ptr, x.size, r.prec).reshape(x.shape) / np.sqrt(x.size)
         assert oracle.check_rel_error(z, x, TOL[r.prec]), (r.describe(), oracle.rel_l2(z, x))
+
+
+def test_micro_batched_stream_vs_fp64(gpu_lib):
+    """300 jobs of the serving stream's generator with sizes 2^10..2^16
+    (both precisions, FFTND 2-D/3-D and FFT1D, both directions), queued at
+    once so they run micro-batched several to a group: every result against
+    an FP64 FFT of its input (numpy's pocketfft, pinned against the direct
+    DFT in test_oracle) within the precision's tolerance, every input
+    unchanged."""
+    reqs = draw_stream(300, seed=47, lo=10, hi=16)
+    srv = api.Server(slots=4, arena_bytes=1 << 30, planners=4)
+    xs, ins, outs, ids = [], [], [], []
+    for k, r in enumerate(reqs):
+        shape = (r.n,) if r.kind == 1 else tuple(r.dims)
+        x = oracle.uniform_input(shape, 7000 + k, DT[r.prec])
+        a, b = api.DeviceBuffer(x.size, r.prec), api.DeviceBuffer(x.size, r.prec)
+        a.upload(x.ravel())
+        xs.append(x)
+        ins.append(a)
+        outs.append(b)
+    # plans first, then every job at once
+    for r, i, o in zip(reqs, ins, outs):
+        sub = srv.submit_1d if r.kind == 1 else srv.submit_nd
+        sub(r.n if r.kind == 1 else r.dims, r.prec, r.direction, i.ptr, o.ptr)
+    srv.drain()
+    srv.reset_stats()
+    for r, i, o in zip(reqs, ins, outs):
+        sub = srv.submit_1d if r.kind == 1 else srv.submit_nd
+        ids.append(sub(r.n if r.kind == 1 else r.dims, r.prec, r.direction, i.ptr, o.ptr))
+    for r, j in zip(reqs, ids):
+        s = srv.wait(j, 60)
+        assert s is not None and s.status == 0, (r.describe(), s and s.message)
+    groups, grouped = srv.batch_stats()
+    srv.close()
+    print(f"{len(reqs)} jobs, {grouped} micro-batched in {groups} groups", flush=True)
+    assert grouped == len(reqs) and groups < len(reqs)
+    worst = {0: 0.0, 1: 0.0}
+    for r, x, i, o in zip(reqs, xs, ins, outs):
+        y = api.download_ptr(o.ptr, x.size, r.prec).reshape(x.shape)
+        xd = x.astype(np.complex128)
+        ref = np.fft.fftn(xd) if r.direction == 0 else np.fft.ifftn(xd) * x.size
+        e = oracle.rel_l2(y, ref)
+        worst[r.prec] = max(worst[r.prec], e)
+        assert oracle.check_rel_error(y, ref, TOL[r.prec]), (r.describe(), e)
+        assert np.array_equal(api.download_ptr(i.ptr, x.size, r.prec).reshape(x.shape), x), r.describe()
+    print(f"worst relL2: complex64 {worst[0]:.2e}, complex128 {worst[1]:.2e}", flush=True)
