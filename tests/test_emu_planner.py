@@ -461,3 +461,35 @@ def test_single_pass_extents(emu_lib):
     assert [q.pass_extents(i) for i in range(q.num_passes())] == [(96, 96)] * q.num_passes()
     with pytest.raises(api.ShafftError):
         q.execute_pass(0, small_a.ptr, 95, small_b.ptr, 96, 0)
+
+
+def test_lds_row_padding_choices(emu_lib):
+    """The bank model's padded / unpadded LDS rows per pass: odd radices and
+    line-fast lanes run unpadded; a k-fast pass whose first stage has a
+    power-of-two radix of 16 keeps the padding; the knob forces either."""
+    L = emu_lib
+
+    def passes(make):
+        return [ln for ln in make().describe().strip().splitlines()]
+
+    nd = lambda dims, prec: (lambda: api.NDBackendPlan(list(dims), list(range(len(dims))), prec, L=L))  # noqa: E731
+    one = lambda n, prec: (lambda: api.OneDBackendPlan(n, prec, L=L))  # noqa: E731
+    p = passes(nd((9, 405), 0))
+    assert "radices=9x9x5" in p[0] and "unpadded" in p[0], p  # odd radices, k-fast
+    p = passes(nd((2, 5488), 0))
+    assert "radices=16x7x7x7" in p[0] and "unpadded" not in p[0], p  # radix-16 first stage, k-fast
+    p = passes(one(6048, 0))
+    assert "radices=16x21x18" in p[0] and "unpadded" not in p[0], p
+    p = passes(one(405 * 405, 0))
+    assert all("unpadded" in ln for ln in p), p  # line-fast and k-fast passes of 9x9x5
+    # fixed-kernel (power-of-two) passes always keep padded rows
+    p = passes(nd((64, 64, 32), 0))
+    assert all("unpadded" not in ln for ln in p if "fixed" in ln), p
+
+
+def test_lds_row_padding_knob(emu_lib, monkeypatch):
+    L = emu_lib
+    for knob, want in (("1", False), ("0", True)):
+        monkeypatch.setenv("SHAFFT_AMD_LDS_PAD", knob)
+        d = api.NDBackendPlan([2, 5488], [0, 1], 0, L=L).describe().strip().splitlines()
+        assert ("unpadded" in d[0]) == want, (knob, d)
