@@ -10,6 +10,7 @@ length is run beside it and must agree to the same tolerance.
 """
 import os
 import subprocess
+import tempfile
 import sys
 
 import numpy as np
@@ -111,8 +112,8 @@ for prec in (api.C2C, api.Z2Z):
     p.synchronize()
     np.save(sys.argv[1] + f'_{prec}.npy', api.download_ptr(b.ptr, n, prec))
 """
-    tmp = os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else "/tmp"
-    stem = os.path.join(tmp, f"blue_generic_{os.getpid()}")
+    tmpdir = tempfile.mkdtemp(prefix="blue_generic_")
+    stem = os.path.join(tmpdir, "out")
     env = dict(os.environ, SHAFFT_AMD_BLUE_FX="0")
     r = subprocess.run([sys.executable, "-c", code, stem], cwd=ROOT, env=env, capture_output=True, text=True,
                        timeout=300)
@@ -126,3 +127,4 @@ for prec in (api.C2C, api.Z2Z):
         e = oracle.rel_l2(out[api.FORWARD], g.astype(np.complex128))
         print(f"prec {prec}: fixed vs generic relL2 {e:.3e}", flush=True)
         assert e <= TOL[prec]
+    os.rmdir(tmpdir)
