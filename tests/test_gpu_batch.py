@@ -105,6 +105,21 @@ def test_batch_full_workload_properties(gpu_lib):
         worst_p, worst_r = max(worst_p, ep), max(worst_r, er)
     print(f"4096 transforms, {tot} elements: worst Parseval {worst_p:.3e}, worst round trip {worst_r:.3e}")
     assert worst_p < 1e-5 and worst_r < 1e-5 * np.sqrt(np.log(2 ** 24))
+    # Every element of a seeded sample of the transforms (the 8 largest and
+    # 88 others, drawn from every part of the work list the per-XCD queues
+    # split) against an FP64 FFT of its input.
+    sizes = np.array([int(np.prod(sh)) for sh in shapes])
+    pick = set(np.argsort(sizes)[-8:].tolist())
+    pick |= set(np.random.default_rng(46).choice(len(shapes), 88, replace=False).tolist())
+    worst = 0.0
+    for i in sorted(pick):
+        sh = shapes[i]
+        s, e = bp.offsets[i], bp.offsets[i] + int(sizes[i])
+        ref = np.fft.fftn(x[s:e].astype(np.complex128).reshape(sh))
+        got = spec[s:e].reshape(sh)
+        worst = max(worst, oracle.rel_l2(got, ref))
+        assert oracle.check_rel_error(got, ref, 1e-5), (i, sh)
+    print(f"{len(pick)} transforms element by element vs FP64: worst relL2 {worst:.3e}")
 
 
 # Split-precision matrix-core stages: bound on the relative L2 error per
