@@ -145,6 +145,9 @@ extern "C" int sa_batch_occupancy(int prec, int block, size_t lds) {
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024);
   (void)hipGetLastError();
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, block, lds) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, block, lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
   return blocks;
 }

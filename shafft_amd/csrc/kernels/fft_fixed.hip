@@ -173,7 +173,10 @@ extern "C" int sa_fixed_occupancy(int n, int prec, int sign, int nthr, size_t ld
   }
 #undef SA_FX_CASE
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)hipGetLastError();
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, nthr, lds) != hipSuccess) {
     (void)hipGetLastError();
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, nthr, lds) != hipSuccess) return 0;
+    return 0;
+  }
   return blocks;
 }

@@ -153,6 +153,7 @@ extern "C" int sa_pass_occupancy(const sa_pass_t* p, int prec, int twl) {
   const size_t lds = ((size_t)p->C * p->ldsstride + (twl ? (size_t)p->tw_lds : 0)) * esz;
   const void* f = prec ? sa::pass_kernel<double>(p, twl != 0) : sa::pass_kernel<float>(p, twl != 0);
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, sa_max_lds_bytes());
+  (void)hipGetLastError();
   int blocks = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, p->C * p->T, lds) != hipSuccess) {
     (void)hipGetLastError();
