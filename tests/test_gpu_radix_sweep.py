@@ -1,0 +1,73 @@
+"""GPU parity over every 7-smooth line length from 2 to 4096 (and a few
+longer ones), so that every radix plan the planner can pick (radices 2-10,
+12, 14, 15, 16, 18, 20, 21, and 24, 25, 28 for complex double), with and
+without register-resident first/last stages and on padded or unpadded LDS
+rows, runs at least once.
+
+Each length runs through the seam (the reference's fftndPlan/fftndExecute,
+src/cpp/fft/hipfft_method/fftnd_method.hpp:14-31) on a 2-D tensor of 5 lines
+in both layouts: (5, n) over axis 1 (unit-stride lines, k-fast lane mapping)
+and (n, 5) over axis 0 (5 adjacent lines at stride 5, line-fast mapping),
+forward and backward, against the FP64 direct DFT for n <= 1024 and an FP64
+FFT above, with check_rel_error at TOL_F / TOL_D
+(tests/include/test_utils.hpp:84-122).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from shafft_amd import api
+
+TOL = {api.C2C: 1e-5, api.Z2Z: 1e-12}
+DT = {api.C2C: np.complex64, api.Z2Z: np.complex128}
+
+
+def smooth7(n: int) -> bool:
+    for p in (2, 3, 5, 7):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+LENGTHS = [n for n in range(2, 4097) if smooth7(n)] + [4725, 6125, 7203, 8192, 10080, 12005]
+
+
+def reference(x, axis, sign):
+    if x.shape[axis] <= 1024:
+        return oracle.dftn(x, (axis,), sign)
+    return np.fft.fft(x, axis=axis) if sign < 0 else np.fft.ifft(x, axis=axis) * x.shape[axis]
+
+
+def sweep(prec, lengths, L=None):
+    worst = {}
+    for n in lengths:
+        for dims, axis in (((5, n), 1), ((n, 5), 0)):
+            x = oracle.uniform_input(dims, 60 + n % 97, DT[prec])
+            x64 = x.astype(np.complex128)
+            p = api.NDBackendPlan(list(dims), [axis], prec, L)
+            a, b = api.DeviceBuffer(x.size, prec, L), api.DeviceBuffer(x.size, prec, L)
+            for direction, sign in ((api.FORWARD, -1), (api.BACKWARD, +1)):
+                a.upload(x.ravel())
+                d, _ = p.execute(a.ptr, b.ptr, direction)
+                got = api.download_ptr(d, x.size, prec, L).reshape(dims)
+                ref = reference(x64, axis, sign)
+                e = oracle.rel_l2(got, ref)
+                key = "k-fast" if axis == 1 else "line-fast"
+                worst[key] = max(worst.get(key, 0.0), e)
+                assert oracle.check_rel_error(got, ref, TOL[prec]), (n, dims, direction, e, p.describe())
+    print(f"prec {prec}: {len(lengths)} lengths, worst relL2 " +
+          ", ".join(f"{k} {v:.3e}" for k, v in worst.items()))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
+def test_every_7smooth_length(gpu_lib, prec):
+    sweep(prec, LENGTHS)
+
+
+# The CPU emulation build replays every pass of the same plans with bounds
+# checks on each LDS and buffer access (every eighth length and two of the
+# longer ones, to keep the CPU suite short).
+@pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
+def test_every_7smooth_length_emulated(emu_lib, prec):
+    sweep(prec, LENGTHS[::8] + [4725, 8192], emu_lib)
