@@ -71,3 +71,37 @@ def test_every_7smooth_length(gpu_lib, prec):
 @pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
 def test_every_7smooth_length_emulated(emu_lib, prec):
     sweep(prec, LENGTHS[::8] + [4725, 8192], emu_lib)
+
+
+def batch_sweep(prec, lengths, L=None):
+    """The same lengths through the batched launch (each pass re-tiled for
+    the batch kernel): one plan of (5, n) and (n, 5) transforms over both
+    axes, forward and backward, each against the FP64 reference."""
+    shapes = [s for n in lengths for s in ((5, n), (n, 5))]
+    bp = api.BatchPlan(shapes, prec, L)
+    xs = [oracle.uniform_input(sh, 70 + i % 89, DT[prec]) for i, sh in enumerate(shapes)]
+    a, o, w = (api.DeviceBuffer(bp.total, prec, L) for _ in range(3))
+    a.upload(np.concatenate([x.ravel() for x in xs]))
+    bp.set_buffers(a.ptr, o.ptr, w.ptr)
+    for direction, sign in ((api.FORWARD, -1), (api.BACKWARD, +1)):
+        bp.execute(direction, use_graph=False)
+        bp.synchronize()
+        y = o.download()
+        worst = 0.0
+        for i, (sh, x) in enumerate(zip(shapes, xs)):
+            got = y[bp.offsets[i]:bp.offsets[i] + x.size].reshape(sh)
+            ref = reference(reference(x.astype(np.complex128), 0, sign), 1, sign)
+            worst = max(worst, oracle.rel_l2(got, ref))
+            assert oracle.check_rel_error(got, ref, TOL[prec]), (sh, direction)
+        print(f"batched prec {prec} dir {direction}: {len(shapes)} transforms, worst relL2 {worst:.3e}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
+def test_every_7smooth_length_batched(gpu_lib, prec):
+    batch_sweep(prec, [n for n in LENGTHS if n <= 4096])
+
+
+@pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
+def test_every_7smooth_length_batched_emulated(emu_lib, prec):
+    batch_sweep(prec, LENGTHS[:-6:8], emu_lib)
