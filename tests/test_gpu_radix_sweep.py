@@ -105,3 +105,45 @@ def test_every_7smooth_length_batched(gpu_lib, prec):
 @pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
 def test_every_7smooth_length_batched_emulated(emu_lib, prec):
     batch_sweep(prec, LENGTHS[:-6:8], emu_lib)
+
+
+# Longer FFT1D lengths (four-step splits into two or three passes, the
+# four-step twiddle fused into the stores): a seeded draw of 7-smooth lengths
+# up to 2^20 plus the powers of two 2^13 .. 2^22.
+def long_lengths(count, hi, seed):
+    pool = [n for n in range(4097, hi + 1) if smooth7(n)]
+    rng = np.random.default_rng(seed)
+    return sorted(set(rng.choice(pool, count, replace=False).tolist()))
+
+
+def fft1d_sweep(prec, lengths, L=None):
+    worst = 0.0
+    for n in lengths:
+        x = oracle.uniform_input((n,), 80 + n % 89, DT[prec])
+        x64 = x.astype(np.complex128)
+        p = api.FFT1D(L)
+        p.init(n, n, 0, prec)
+        p.plan()
+        a, b = api.DeviceBuffer(n, prec, L), api.DeviceBuffer(n, prec, L)
+        for direction, sign in ((api.FORWARD, -1), (api.BACKWARD, +1)):
+            a.upload(x)
+            p.set_buffers(a.ptr, b.ptr)
+            p.execute(direction)
+            d, _ = p.get_buffers()
+            got = api.download_ptr(d, n, prec, L)
+            ref = np.fft.fft(x64) if sign < 0 else np.fft.ifft(x64) * n
+            worst = max(worst, oracle.rel_l2(got, ref))
+            assert oracle.check_rel_error(got, ref, TOL[prec]), (n, direction)
+        p.release()
+    print(f"FFT1D prec {prec}: {len(lengths)} lengths {lengths[0]}..{lengths[-1]}, worst relL2 {worst:.3e}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
+def test_fft1d_long_7smooth_lengths(gpu_lib, prec):
+    fft1d_sweep(prec, long_lengths(48, 1 << 20, 81) + [1 << k for k in range(13, 23)])
+
+
+@pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
+def test_fft1d_long_7smooth_lengths_emulated(emu_lib, prec):
+    fft1d_sweep(prec, long_lengths(6, 1 << 16, 82), emu_lib)
