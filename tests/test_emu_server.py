@@ -4,6 +4,7 @@ The emulated launches complete synchronously, so these tests cover the
 request loop's bookkeeping (plan cache, arena admission, job states, error
 paths, statistics) and that a served job computes exactly what the plain
 backend call computes; overlap on a real device is covered by the GPU test.
+The concurrent-client check also runs on the GPU (marked gpu).
 """
 import numpy as np
 import pytest
@@ -260,3 +261,61 @@ def test_stream_generator():
     for r in reqs[:200]:
         if r.kind == 0:
             assert int(np.prod(r.dims)) == r.elems and len(r.dims) in (2, 3)
+
+
+def concurrent_clients(L, clients=4, rounds=3):
+    """Several host threads submit and wait on one server at once (the
+    submit/wait entry points are called without Python's lock held); every
+    client's results equal those of one client on a one-slot server, bit for
+    bit, and the statistics count every job once."""
+    import threading
+
+    xs, ins = _inputs(L)
+    alone = api.Server(slots=1, arena_bytes=8 << 20, planners=1, L=L)
+    ref = []
+    for job, i, x in zip(MIX, ins, xs):
+        o = api.DeviceBuffer(x.size, job[2], L)
+        assert alone.wait(_submit(alone, job, i.ptr, o.ptr), 120).status == 0
+        ref.append(api.download_ptr(o.ptr, x.size, job[2], L))
+    alone.close()
+    srv = api.Server(slots=4, arena_bytes=64 << 20, planners=2, L=L)
+    outs = [[[api.DeviceBuffer(x.size, j[2], L) for x, j in zip(xs, MIX)] for _ in range(rounds)]
+            for _ in range(clients)]
+    errors = []
+
+    def client(c):
+        try:
+            for r in range(rounds):
+                order = list(range(len(MIX)))[c % len(MIX):] + list(range(len(MIX)))[:c % len(MIX)]
+                ids = [(k, _submit(srv, MIX[k], ins[k].ptr, outs[c][r][k].ptr)) for k in order]
+                for k, jid in ids:
+                    res = srv.wait(jid, 120)
+                    if res is None or res.status != 0:
+                        errors.append((c, r, MIX[k], res and res.message))
+        except Exception as e:  # reported below, in the main thread
+            errors.append((c, repr(e)))
+
+    ts = [threading.Thread(target=client, args=(c,)) for c in range(clients)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(300)
+    assert not any(t.is_alive() for t in ts), "a client did not finish"
+    assert not errors, errors
+    for c in range(clients):
+        for r in range(rounds):
+            for k, (x, job) in enumerate(zip(xs, MIX)):
+                y = api.download_ptr(outs[c][r][k].ptr, x.size, job[2], L)
+                assert np.array_equal(y, ref[k]), (c, r, job)
+    st = srv.stats()
+    assert st["completed"] == clients * rounds * len(MIX) and st["failed"] == 0
+    srv.close()
+
+
+def test_concurrent_clients(emu_lib):
+    concurrent_clients(emu_lib)
+
+
+@pytest.mark.gpu
+def test_concurrent_clients_gpu(gpu_lib):
+    concurrent_clients(None, clients=8, rounds=4)
