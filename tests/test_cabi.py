@@ -1,8 +1,11 @@
 """The product library loads without a GPU and exports every entry point
-declared in include/shafft_amd.h.  No compute call is made."""
+declared in include/shafft_amd.h.  No compute call is made.  Without the
+library every entry point raises."""
 import ctypes as ct
 import os
 import re
+
+import pytest
 
 from shafft_amd import api
 
@@ -130,3 +133,19 @@ def test_synthetic_input_matches_oracle_recipe(emu_lib):
         assert np.array_equal(g, oracle.gaussian_input((1000,), 44, dt))
     assert emu_lib.shafftAmdSyntheticInput(None, 4, 1, 0, 0) != 0
     assert emu_lib.shafftAmdSyntheticInput(None, 0, 1, 0, 0) == 0
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """No fallback: without libshafft_amd.so every entry point raises instead
+    of computing anything elsewhere."""
+    monkeypatch.setattr(api, "_LIB", None)
+    monkeypatch.setattr(api, "_CURRENT", None)
+    monkeypatch.setattr(api, "library_path", lambda: str(tmp_path / "libshafft_amd.so"))
+    with pytest.raises(ImportError):
+        api.lib()
+    with pytest.raises(ImportError):
+        api.FFTND()
+    with pytest.raises(ImportError):
+        api.OneDBackendPlan(1024, api.C2C)
+    with pytest.raises(ImportError):
+        api.Server()
