@@ -228,9 +228,19 @@ std::vector<int> radix_plan(int64_t n, int maxrad) {
   return r;
 }
 
-// Radix sequence the fixed-length kernel is compiled with (fft_fixed.hip,
-// Radices<N>); its twiddle table must follow the same sequence.
-static std::vector<int> fixed_radix_plan(int64_t n) {
+// Radix sequence the fixed-length kernels are compiled with
+// (fixed_stages.hpp, Plan<N, E>); the twiddle table must follow the same
+// sequence.  With 8 elements per thread (Bluestein rows) every radix
+// divides 8.
+static std::vector<int> fixed_radix_plan(int64_t n, int elems = 16) {
+  if (elems == 8) {
+    switch (n) {
+      case 1024: return {8, 8, 4, 4};
+      case 2048: return {8, 8, 8, 4};
+      case 4096: return {8, 8, 8, 8};
+      default: return {};
+    }
+  }
   switch (n) {
     case 32: return {8, 4};
     case 64: return {8, 8};
@@ -1077,7 +1087,17 @@ bool PassProgram::configure_blue(PassSpec& ps, int kind) {
   } else {
     C = std::max(1, env_int("SHAFFT_AMD_BLUE_ROWT", 256) / (n / 16));
   }
-  const int E = sa_blue_elems(n, prec_, kind, C);
+  // Complex-float row passes with fewer rows than CUs take 8 elements per
+  // thread (twice the threads per row, one more stage): 16 / 32 / 64 rows
+  // of 4096 points 8.9 -> 8.0, 9.0 -> 8.2, 9.2 -> 8.2 us; with a row for
+  // every CU the extra stage cancels the gain (1,000,003 points, 512 rows:
+  // 15.5 / 15.8 us), and complex-double rows gain nothing (40,009 points:
+  // 11.1 / 11.3 us).  SHAFFT_AMD_BLUE_ROW_E=16/8 forces either form.
+  int E = sa_blue_elems(n, prec_, kind, C);
+  if (!cols && n <= 4096) {
+    const int knob = env_int("SHAFFT_AMD_BLUE_ROW_E", 0);
+    if (knob == 8 || (knob == 0 && prec_ == 0 && p.batch < device_cus())) E = 8;
+  }
   const int T = n / E;
   int logC = 0;
   while ((1 << (logC + 1)) <= C) ++logC;
@@ -1087,7 +1107,8 @@ bool PassProgram::configure_blue(PassSpec& ps, int kind) {
   const int budget = lds_budget_bytes();
   while (logC > 0 && (((int64_t)T << logC) > 512 || ((int64_t)stride << logC) * esz > budget)) --logC;
   if (((int64_t)T << logC) > 512 || ((int64_t)stride << logC) * esz > budget) return false;
-  const std::vector<int> rad = fixed_radix_plan(n);
+  const std::vector<int> rad = fixed_radix_plan(n, E);
+  if (rad.empty()) return false;
   std::vector<int> offs;
   p.tw = stage_table(rad, offs);
   p.nstages = (int32_t)rad.size();

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(512) void blue_cols(const sa_pass_t a) {
 
   if constexpr (HK == HK_CHIRP_IN) {
     // The four-step twiddle's table lookups go out before the stages.
-    const PtwRaw<R> q = ptw_fetch<R, N, E / StageInfo<N, Radices<N>::n - 1>::R, E>(a, b, tl);
+    const PtwRaw<R> q = ptw_fetch<R, N, E / StageInfo<N, Plan<N, E>::n - 1, E>::R, E>(a, b, tl);
     mids<R, N, SIGN, 1, E>(line, a, twb, tl);
     __syncthreads();
     stage_last<R, N, SIGN, true, E>(a, twb, line, tl, store_ctx_from<R, N, E>(a, b, tl, q));
@@ -137,9 +137,11 @@ __global__ __launch_bounds__(512) void blue_cols(const sa_pass_t a) {
 // filter are loaded together, the forward DFT's outputs stay in the
 // registers of the thread that multiplies them by the filter and starts the
 // inverse DFT from them (element tl + T*m in slot m on both sides).
-template <typename R, int N, bool TWL>
+// E: 16 elements per thread, or 8 (twice the threads per row, radix-8/4
+// plans) for rows of 1024..4096 points.
+template <typename R, int N, bool TWL, int E>
 __global__ __launch_bounds__(512) void blue_rows(const sa_pass_t a) {
-  constexpr int E = 16, T = N / E;
+  constexpr int T = N / E;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(lds_raw);
   const int logC = a.logC;
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(512) void blue_rows(const sa_pass_t a) {
 
   // Table lookups of the store's conjugate four-step twiddle, issued with
   // the row's loads.
-  const PtwRaw<R> q = ptw_fetch<R, N, E / StageInfo<N, Radices<N>::n - 1>::R, E>(a, b, tl);
+  const PtwRaw<R> q = ptw_fetch<R, N, E / StageInfo<N, Plan<N, E>::n - 1, E>::R, E>(a, b, tl);
   cpx<R> v[E], h[E];
   {
     const Line l = line_of(a, have ? b : 0u);
@@ -202,17 +204,17 @@ static hipError_t launch_cols_t(const sa_pass_t* p, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename R, int N, bool TWL>
+template <typename R, int N, bool TWL, int E>
 static hipError_t launch_rows_t(const sa_pass_t* p, hipStream_t s) {
   static const bool attr = [] {
-    set_lds_attr(&blue_rows<R, N, TWL>);
+    set_lds_attr(&blue_rows<R, N, TWL, E>);
     return true;
   }();
   (void)attr;
-  const uint32_t nthr = (uint32_t)((N / 16) << p->logC);
+  const uint32_t nthr = (uint32_t)((N / E) << p->logC);
   const uint64_t tiles = ((uint64_t)p->batch + (1u << p->logC) - 1) >> p->logC;
   const size_t lds = (((size_t)p->ldsstride << p->logC) + (TWL ? (size_t)p->tw_lds : 0)) * sizeof(cpx<R>);
-  hipLaunchKernelGGL((blue_rows<R, N, TWL>), dim3((uint32_t)tiles), dim3(nthr), lds, s, *p);
+  hipLaunchKernelGGL((blue_rows<R, N, TWL, E>), dim3((uint32_t)tiles), dim3(nthr), lds, s, *p);
   return hipGetLastError();
 }
 
@@ -235,7 +237,11 @@ static hipError_t launch_cols(const sa_pass_t* p, int hook, hipStream_t s) {
 
 template <typename R, int N>
 static hipError_t launch_rows(const sa_pass_t* p, hipStream_t s) {
-  return p->tw_lds > 0 ? launch_rows_t<R, N, true>(p, s) : launch_rows_t<R, N, false>(p, s);
+  if constexpr (N <= 4096)
+    if (p->emax == 8)
+      return p->tw_lds > 0 ? launch_rows_t<R, N, true, 8>(p, s) : launch_rows_t<R, N, false, 8>(p, s);
+  if (p->emax != 16) return hipErrorInvalidValue;
+  return p->tw_lds > 0 ? launch_rows_t<R, N, true, 16>(p, s) : launch_rows_t<R, N, false, 16>(p, s);
 }
 
 template <typename R>

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(512) void fft_fixed_pass(const sa_pass_t a) {
     __syncthreads();
     if (same) {
       const StoreCtx<R> ctx =
-          store_ctx<R, N, E / StageInfo<N, Radices<N>::n - 1>::R>(a, (tile << logC) + c, tl);
+          store_ctx<R, N, E / StageInfo<N, Plan<N, E>::n - 1, E>::R>(a, (tile << logC) + c, tl);
       stage_last<R, N, SIGN, true>(a, twb, line, tl, ctx);
     } else {
       // Transposing pass: natural order through LDS, then the store mapping.

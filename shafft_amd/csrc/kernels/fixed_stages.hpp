@@ -36,13 +36,22 @@ template <> struct Radices<2048> { static constexpr int n = 3, r[4] = {16, 16, 8
 template <> struct Radices<4096> { static constexpr int n = 3, r[4] = {16, 16, 16, 0}; };
 template <> struct Radices<8192> { static constexpr int n = 4, r[4] = {16, 16, 8, 4}; };
 
-template <int N, int S>
+// The plan a kernel with E elements per thread runs: the radices above for
+// E >= 16; with 8 elements per thread (the Bluestein row passes' wider
+// workgroups) every radix divides 8.
+template <int N, int E>
+struct Plan : Radices<N> {};
+template <> struct Plan<1024, 8> { static constexpr int n = 4, r[4] = {8, 8, 4, 4}; };
+template <> struct Plan<2048, 8> { static constexpr int n = 4, r[4] = {8, 8, 8, 4}; };
+template <> struct Plan<4096, 8> { static constexpr int n = 4, r[4] = {8, 8, 8, 8}; };
+
+template <int N, int S, int E = 16>
 struct StageInfo {
-  static constexpr int R = Radices<N>::r[S];
-  static constexpr int NS = S == 0 ? 1 : StageInfo<N, S - 1>::NS * StageInfo<N, S - 1>::R;
+  static constexpr int R = Plan<N, E>::r[S];
+  static constexpr int NS = S == 0 ? 1 : StageInfo<N, S - 1, E>::NS * StageInfo<N, S - 1, E>::R;
 };
-template <int N>
-struct StageInfo<N, -1> {
+template <int N, int E>
+struct StageInfo<N, -1, E> {
   static constexpr int R = 1, NS = 1;
 };
 
@@ -239,7 +248,7 @@ __device__ __forceinline__ void put(const sa_pass_t& a, const StoreCtx<R>& c, in
 template <typename R, int N, int SIGN, int E = Elems<R, N>::E>
 __device__ __forceinline__ void stage_first(const cpx<R> (&v)[E], cpx<R>* line,
                                             uint32_t tl) {
-  constexpr int RAD = Radices<N>::r[0];
+  constexpr int RAD = Plan<N, E>::r[0];
   constexpr int T = N / E, MAXB = E / RAD;
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
@@ -257,7 +266,7 @@ __device__ __forceinline__ void stage_first(const cpx<R> (&v)[E], cpx<R>* line,
 template <typename R, int N, int S, int SIGN, int E = Elems<R, N>::E>
 __device__ __forceinline__ void stage_mid(cpx<R>* line, const cpx<R>* __restrict__ tw,
                                           uint32_t tl) {
-  constexpr int RAD = StageInfo<N, S>::R, NS = StageInfo<N, S>::NS;
+  constexpr int RAD = StageInfo<N, S, E>::R, NS = StageInfo<N, S, E>::NS;
   constexpr int T = N / E, MAXB = E / RAD, NB = N / RAD;
   cpx<R> u[MAXB][RAD];
 #pragma unroll
@@ -286,7 +295,7 @@ __device__ __forceinline__ void stage_mid(cpx<R>* line, const cpx<R>* __restrict
 template <typename R, int N, int SIGN, int S, int E = Elems<R, N>::E>
 __device__ __forceinline__ void mids(cpx<R>* line, const sa_pass_t& a, const cpx<R>* twb,
                                      uint32_t tl) {
-  if constexpr (S < Radices<N>::n - 1) {
+  if constexpr (S < Plan<N, E>::n - 1) {
     __syncthreads();
     stage_mid<R, N, S, SIGN, E>(line, twb + a.twoff[S], tl);
     mids<R, N, SIGN, S + 1, E>(line, a, twb, tl);
@@ -299,8 +308,8 @@ __device__ __forceinline__ void mids(cpx<R>* line, const sa_pass_t& a, const cpx
 template <typename R, int N, int SIGN, bool STORE, int E = Elems<R, N>::E>
 __device__ __forceinline__ void stage_last(const sa_pass_t& a, const cpx<R>* twb, cpx<R>* line,
                                            uint32_t tl, const StoreCtx<R>& c) {
-  constexpr int S = Radices<N>::n - 1;
-  constexpr int RAD = StageInfo<N, S>::R;
+  constexpr int S = Plan<N, E>::n - 1;
+  constexpr int RAD = StageInfo<N, S, E>::R;
   constexpr int T = N / E, MAXB = E / RAD, NB = N / RAD;
   const cpx<R>* tw = twb + a.twoff[S];
   if constexpr (STORE) {
@@ -351,8 +360,8 @@ template <typename R, int N, int SIGN, int E = Elems<R, N>::E>
 __device__ __forceinline__ void stage_last_regs(const sa_pass_t& a, const cpx<R>* twb,
                                                 const cpx<R>* line, uint32_t tl,
                                                 cpx<R> (&out)[E]) {
-  constexpr int S = Radices<N>::n - 1;
-  constexpr int RAD = StageInfo<N, S>::R;
+  constexpr int S = Plan<N, E>::n - 1;
+  constexpr int RAD = StageInfo<N, S, E>::R;
   constexpr int T = N / E, MAXB = E / RAD, NB = N / RAD;
   const cpx<R>* tw = twb + a.twoff[S];
 #pragma unroll

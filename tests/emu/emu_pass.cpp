@@ -338,7 +338,8 @@ extern "C" int sa_launch_blue(const sa_pass_t* p, int prec, int kind, void*) {
     std::abort();
   };
   if (!sa_blue_supported(p->n, kind)) die("length");
-  if (p->emax != sa_blue_elems(p->n, prec, kind, p->C)) die("elements per thread");
+  const bool row8 = kind == SA_BLUE_ROWS && p->n <= 4096 && p->emax == 8;  // the 8-element row variant
+  if (p->emax != sa_blue_elems(p->n, prec, kind, p->C) && !row8) die("elements per thread");
   if (p->T * p->emax != p->n) die("T");
   if (p->C != (1 << p->logC)) die("C");
   if (p->C * p->T > 512) die("threads");
@@ -356,7 +357,9 @@ extern "C" int sa_launch_blue(const sa_pass_t* p, int prec, int kind, void*) {
         {32, {8, 4}},       {64, {8, 8}},       {128, {16, 8}},     {256, {16, 16}},
         {512, {8, 8, 8}},   {1024, {16, 8, 8}}, {2048, {16, 16, 8}}, {4096, {16, 16, 16}},
         {8192, {16, 16, 8, 4}}};
-    const auto& pl = plans.at(p->n);
+    static const std::map<int, std::vector<int>> plans8 = {
+        {1024, {8, 8, 4, 4}}, {2048, {8, 8, 8, 4}}, {4096, {8, 8, 8, 8}}};
+    const auto& pl = row8 ? plans8.at(p->n) : plans.at(p->n);
     if ((int)pl.size() != p->nstages) die("stage count differs from the kernel's plan");
     for (int s = 0; s < p->nstages; ++s)
       if (pl[s] != p->radix[s]) die("radix differs from the kernel's plan");

@@ -128,3 +128,25 @@ for prec in (api.C2C, api.Z2Z):
         print(f"prec {prec}: fixed vs generic relL2 {e:.3e}", flush=True)
         assert e <= TOL[prec]
     os.rmdir(tmpdir)
+
+
+@pytest.mark.parametrize("prec", [api.C2C, api.Z2Z])
+@pytest.mark.parametrize("n", [5003, 12007, 40009, 1000003])
+def test_bluestein_row_forms(gpu_lib, monkeypatch, n, prec):
+    """Both forms of the filter row pass, forced (16 elements per thread,
+    radix-16 stages; 8 elements per thread, radix-8/4 stages, twice the
+    threads): each against numpy's FFT, and the two within the tolerance of
+    each other."""
+    x = oracle.gaussian_input((n,), 45, DT[prec])
+    x128 = x.astype(np.complex128)
+    outs = {}
+    for e in ("16", "8"):
+        monkeypatch.setenv("SHAFFT_AMD_BLUE_ROW_E", e)
+        desc, out = _run_1d(n, prec, x)
+        assert "blue2" in desc, desc
+        outs[e] = out
+        for direction, y in out.items():
+            ref = np.fft.fft(x128) if direction == api.FORWARD else np.fft.ifft(x128) * n
+            assert oracle.check_rel_error(y, ref, TOL[prec]), (n, prec, e, direction)
+    for direction in outs["16"]:
+        assert oracle.check_rel_error(outs["8"][direction], outs["16"][direction], TOL[prec])
