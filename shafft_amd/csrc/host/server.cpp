@@ -472,7 +472,7 @@ void Server::scheduler() {
   (void)hipSetDevice(device_);
   std::unique_lock<std::mutex> lk(mu_);
   std::vector<int64_t> admitted;
-  std::vector<int> admitted_mb;  // slots of the micro-batches formed this round
+  std::vector<int> admitted_mb;  // slots of the micro-batches formed in this loop iteration
   for (;;) {
     bool progress = false;
     // Retire: a slot's job is done when its end event has completed.
