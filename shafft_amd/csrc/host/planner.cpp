@@ -669,8 +669,22 @@ static bool configure_fixed(sa_pass_t& p, int prec, int64_t& grid) {
   const int occ = sa_fixed_occupancy(n, prec, -1, nthr, lds);
   if (occ <= 0) return false;  // p keeps the generic kernel's launch shape
   p.tw_lds = tw_lds;
-  p.xcd_order = env_int("SHAFFT_AMD_XCD", 0) ? 1 : 0;
   const int64_t tiles = (p.batch + (1 << logC) - 1) >> logC;
+  {
+    // Tile order.  Default: passes strided by at least 2^16 elements on both
+    // sides walk rows of 32 tiles down the rows first, so the tiles in
+    // flight spread over many rows of each plane instead of one 64 KiB
+    // window (512^3 complex128 z pass 4966 -> 5062 GB/s, FFT1D 2^27
+    // complex64 first pass 4289 -> 4436; passes of smaller stride lose and
+    // keep the plain order).  SHAFFT_AMD_XCD=0: plain order everywhere;
+    // =1: each XCD walks its own contiguous eighth of the tiles; =2: rows
+    // of SHAFFT_AMD_XCD_ROW tiles (default 32) on every pass.
+    const int knob = env_int("SHAFFT_AMD_XCD", -1);
+    const int64_t rlen = knob == 2 ? env_int("SHAFFT_AMD_XCD_ROW", 32) : 32;
+    const bool wide = lf && !p.ld_kfast && !p.st_kfast && p.iks >= (1 << 16) && p.oks >= (1 << 16);
+    const bool rows = knob == 2 || (knob < 0 && wide);
+    p.xcd_order = knob == 1 ? 1 : (rows && rlen >= 2 && tiles % rlen == 0) ? (int32_t)rlen : 0;
+  }
   grid = std::min<int64_t>(tiles, (int64_t)occ * device_cus());
   p.logC = logC;
   p.C = 1 << logC;

@@ -49,30 +49,37 @@ __global__ __launch_bounds__(512) void fft_fixed_pass(const sa_pass_t a) {
   // 8 XCDs round robin, so each XCD sees every 8th tile), or with xcd_order
   // the workgroups of XCD x = b % 8 walk their own contiguous eighth.
   uint32_t tile = blockIdx.x, step = gridDim.x, end = tiles;
-  if (a.xcd_order && (gridDim.x & 7u) == 0) {
+  if (a.xcd_order == 1 && (gridDim.x & 7u) == 0) {
     const uint32_t x = blockIdx.x & 7u, per = (tiles + 7u) >> 3;
     tile = x * per + (blockIdx.x >> 3);
     step = gridDim.x >> 3;
     end = min(tiles, (x + 1u) * per);
   }
+  // xcd_order >= 2: the tiles form rows of rlen = xcd_order tiles and are
+  // walked down the rows first, so the tiles in flight at once spread over
+  // many rows (the planner sets it only when rlen divides the tile count;
+  // t -> (t % rows) * rlen + t / rows is then a bijection on the tiles).
+  const bool cm = a.xcd_order >= 2;
+  const uint32_t rlen = cm ? (uint32_t)a.xcd_order : 1u, rows = cm ? tiles / rlen : 1u;
+  auto at = [&](uint32_t t) { return cm ? (t % rows) * rlen + t / rows : t; };
   cpx<R> v[E];
-  if (tile < end) load_tile<R, N>(a, (tile << logC) + c, tl, v);
+  if (tile < end) load_tile<R, N>(a, (at(tile) << logC) + c, tl, v);
   while (tile < end) {
     stage_first<R, N, SIGN>(v, line, tl);
     const uint32_t next = tile + step;
-    if (next < end) load_tile<R, N>(a, (next << logC) + c, tl, v);  // prefetch
+    if (next < end) load_tile<R, N>(a, (at(next) << logC) + c, tl, v);  // prefetch
     mids<R, N, SIGN, 1>(line, a, twb, tl);
     __syncthreads();
     if (same) {
       const StoreCtx<R> ctx =
-          store_ctx<R, N, E / StageInfo<N, Plan<N, E>::n - 1, E>::R>(a, (tile << logC) + c, tl);
+          store_ctx<R, N, E / StageInfo<N, Plan<N, E>::n - 1, E>::R>(a, (at(tile) << logC) + c, tl);
       stage_last<R, N, SIGN, true>(a, twb, line, tl, ctx);
     } else {
       // Transposing pass: natural order through LDS, then the store mapping.
       const StoreCtx<R> dummy = {};
       stage_last<R, N, SIGN, false>(a, twb, line, tl, dummy);
       __syncthreads();
-      const StoreCtx<R> ctx = store_ctx<R, N, 1>(a, (tile << logC) + sc, stl);
+      const StoreCtx<R> ctx = store_ctx<R, N, 1>(a, (at(tile) << logC) + sc, stl);
       if (ctx.ok) {
         const cpx<R>* sl = lds + (size_t)sc * stride;
         cpx<double> w = ctx.w0;

@@ -57,7 +57,7 @@ typedef struct {
   int32_t nt;        /* non-temporal global loads/stores of the data     */
   int32_t logC;      /* log2(C) for the fixed-length kernel (C a power of 2) */
   int32_t tw_lds;    /* fixed kernel: stage-twiddle elements staged in LDS (0: global) */
-  int32_t xcd_order; /* fixed kernel: 1 = each XCD walks its own contiguous tile range */
+  int32_t xcd_order; /* fixed kernel: 1 = each XCD walks its own contiguous tile range; >= 2 = rows of that many tiles walked down the rows first */
   int32_t lds_pad;   /* generic kernel: 1 = LDS rows padded k -> k + k/16, 0 = unpadded */
   sa_fastdiv_t Tdiv, Cdiv, n1div, n2div;
 
