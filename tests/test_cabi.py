@@ -149,3 +149,16 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         api.OneDBackendPlan(1024, api.C2C)
     with pytest.raises(ImportError):
         api.Server()
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py runs one process on one GPU: --gpus above 1 exits with a
+    message (before touching a device)."""
+    import json
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
