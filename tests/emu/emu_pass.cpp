@@ -298,6 +298,11 @@ extern "C" int sa_launch_fixed(const sa_pass_t* p, int prec, int sign, int grid,
   if (p->ldsstride < p->n + (p->n - 1) / 16) die("stride");
   if (!p->lds_pad) die("fixed kernel rows are padded");
   if (grid < 1) die("grid");
+  if (p->xcd_order < 0) die("xcd_order");
+  if (p->xcd_order >= 2) {  // rows of xcd_order tiles: the walk is a bijection only if they tile the pass
+    const int64_t tiles = (p->batch + p->C - 1) / p->C;
+    if (tiles % p->xcd_order != 0) die("tile rows do not divide the tile count");
+  }
   if (p->pre || p->post || p->blu_h || p->ld_level >= 0 || p->st_level >= 0) die("hooks");
   {
     // the kernel's compile-time radix sequence (fft_fixed.hip Radices<N>)
